@@ -1,0 +1,118 @@
+// libb3dhip: one level of the batched radial recursion (reference blocksvibe/recursion_gpu.py:438-510,
+// cauchy.py:15-87, smallgemm.py:14-88), as three gfx950 kernels:
+//
+//   k_cauchy   H[h, n] = h_inf[g(h), n] + sum_i x_i * 2 / (D'_h - D_i), x_i = res[col[g, i], n]:
+//              gathers the prime-pole residues straight from the slot buffer (no staging copy), aligns
+//              them per column, divides by the one-limb integer D' - D_i, sums exactly, rounds once and
+//              adds the h_inf entry;
+//   k_pole_r   Y[t, b0, a, k] = sum_b Ralign[R_t][b0][b] * H[hrow_t][a, b][k]
+//   k_pole_l   res[slot_t][a0, b0][k] = sum_a QLalign[qL_t][a0][a] * Y[t, b0, a, k]
+//
+// Layouts (limb planes, entries innermost = right-hand side k):
+//   slot buffer res: (L, nslots, S*S, B), H: (L, nH, S*S, B), Y: (L, T, S, S, B),
+//   Ralign / QLalign: (L, ntab, S, S) mantissas aligned per row, (ntab, S) row exponents.
+#include "common.hpp"
+#include "mp.hpp"
+#include "../../include/b3dhip.h"
+
+static inline unsigned grid_for(int64_t n, int th) {
+  int64_t b = (n + th - 1) / th;
+  if (b > (1 << 22)) b = 1 << 22;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+template <int L>
+__global__ __launch_bounds__(256) void k_cauchy(const int32_t* __restrict__ rm, const i64* __restrict__ re, i64 rps,
+                                                i64 N, const int32_t* __restrict__ hm, const i64* __restrict__ he,
+                                                i64 hps, const int64_t* __restrict__ ghrow,
+                                                const int32_t* __restrict__ rg, const int32_t* __restrict__ rd, i64 nH,
+                                                const int64_t* __restrict__ col, const int32_t* __restrict__ cold,
+                                                const int32_t* __restrict__ ncol, int Kc, int32_t* __restrict__ om,
+                                                i64* __restrict__ oe, i64 ops) {
+  for (i64 t = blockIdx.x * (i64)blockDim.x + threadIdx.x; t < nH * N; t += (i64)gridDim.x * blockDim.x) {
+    const i64 n = t % N, h = t / N;
+    const int g = rg[h];
+    const int D = rd[h];
+    u32 r[L];
+    const i64 er = mp::cauchy_entry<L>(rm, re, rps, N, n, col + (i64)g * Kc, cold + (i64)g * Kc, ncol[g], D, hm, he,
+                                       hps, ghrow[g] * N + n, r);
+    mp::store(r, om, ops, t);
+    oe[t] = er;
+  }
+}
+
+template <int L>
+__global__ __launch_bounds__(256) void k_pole_r(const int32_t* __restrict__ Hm, const i64* __restrict__ He, i64 hps,
+                                                const int64_t* __restrict__ thrw, const int32_t* __restrict__ tR,
+                                                i64 T, int S, i64 B, const int32_t* __restrict__ Rm,
+                                                const i64* __restrict__ Re, i64 rps, int32_t* __restrict__ Ym,
+                                                i64* __restrict__ Ye, i64 yps) {
+  const i64 SB = (i64)S * B;
+  for (i64 t = blockIdx.x * (i64)blockDim.x + threadIdx.x; t < T * S * SB; t += (i64)gridDim.x * blockDim.x) {
+    const i64 k = t % B;
+    const i64 a = (t / B) % S;
+    const i64 b0 = (t / SB) % S;
+    const i64 task = t / (S * SB);
+    const i64 r = tR[task];
+    const i64 xbase = thrw[task] * S * SB + a * SB + k;  // H[hrow][a*S + b][k], b stride B
+    u32 out[L];
+    const i64 e = mp::small_dot<L>(Rm, rps, (r * S + b0) * S, Re[r * S + b0], Hm, He, hps, xbase, B, S, out);
+    mp::store(out, Ym, yps, t);  // (task, b0, a, k)
+    Ye[t] = e;
+  }
+}
+
+template <int L>
+__global__ __launch_bounds__(256) void k_pole_l(const int32_t* __restrict__ Ym, const i64* __restrict__ Ye, i64 yps,
+                                                const int32_t* __restrict__ tQL, const int64_t* __restrict__ tslot,
+                                                i64 T, int S, i64 B, const int32_t* __restrict__ Qm,
+                                                const i64* __restrict__ Qe, i64 qps, int32_t* __restrict__ rm,
+                                                i64* __restrict__ re, i64 rps) {
+  const i64 SB = (i64)S * B;
+  for (i64 t = blockIdx.x * (i64)blockDim.x + threadIdx.x; t < T * S * SB; t += (i64)gridDim.x * blockDim.x) {
+    const i64 k = t % B;
+    const i64 b0 = (t / B) % S;
+    const i64 a0 = (t / SB) % S;
+    const i64 task = t / (S * SB);
+    const i64 q = tQL[task];
+    const i64 xbase = (task * S + b0) * SB + k;  // Y[task][b0][a][k], a stride B
+    u32 out[L];
+    const i64 e = mp::small_dot<L>(Qm, qps, (q * S + a0) * S, Qe[q * S + a0], Ym, Ye, yps, xbase, B, S, out);
+    const i64 o = tslot[task] * S * SB + (a0 * S + b0) * B + k;
+    mp::store(out, rm, rps, o);
+    re[o] = e;
+  }
+}
+
+extern "C" {
+
+int b3d_cauchy_level(int L, const int32_t* rm, const int64_t* re, int64_t rps, int64_t N, const int32_t* hm,
+                     const int64_t* he, int64_t hps, const int64_t* ghrow, const int32_t* rg, const int32_t* rd,
+                     int64_t nH, const int64_t* col, const int32_t* cold, const int32_t* ncol, int Kc, int32_t* om,
+                     int64_t* oe, int64_t ops, void* stream) {
+  if (nH <= 0 || N <= 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  B3D_DISPATCH_L(L, k_cauchy<LL><<<dim3(grid_for(nH * N, 256)), dim3(256), 0, s>>>(rm, re, rps, N, hm, he, hps, ghrow, rg, rd, nH, col, cold, ncol, Kc, om, oe, ops));
+  return b3d::check_launch("b3d_cauchy_level");
+}
+
+int b3d_pole_level(int L, int S, int64_t B, int64_t T, const int32_t* Hm, const int64_t* He, int64_t hps,
+                   const int64_t* thrw, const int32_t* tR, const int32_t* tQL, const int64_t* tslot,
+                   const int32_t* Rm, const int64_t* Re, int64_t rtps, const int32_t* Qm, const int64_t* Qe,
+                   int64_t qps, int32_t* Ym, int64_t* Ye, int64_t yps, int32_t* rm, int64_t* re, int64_t rps,
+                   void* stream) {
+  if (T <= 0 || B <= 0) return 0;
+  if (S < 1 || S > 16) {
+    b3d::set_error("b3d_pole_level: structure count S must be 1..16");
+    return 2;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t n = T * S * S * B;
+  B3D_DISPATCH_L(L, k_pole_r<LL><<<dim3(grid_for(n, 256)), dim3(256), 0, s>>>(Hm, He, hps, thrw, tR, T, S, B, Rm, Re, rtps, Ym, Ye, yps));
+  if (b3d::check_launch("b3d_pole_level (R)")) return 1;
+  B3D_DISPATCH_L(L, k_pole_l<LL><<<dim3(grid_for(n, 256)), dim3(256), 0, s>>>(Ym, Ye, yps, tQL, tslot, T, S, B, Qm, Qe, qps, rm, re, rps));
+  return b3d::check_launch("b3d_pole_level (L)");
+}
+
+}  // extern "C"

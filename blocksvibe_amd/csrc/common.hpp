@@ -1,0 +1,46 @@
+// Shared host-side plumbing of libb3dhip: error reporting and the limb-count dispatch.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+namespace b3d {
+void set_error(const std::string& msg);
+int check_launch(const char* what);
+}  // namespace b3d
+
+// Limb counts with compiled kernels.  L = 2 ceil(precision / 64) + 2 (precision rounded up to whole
+// 64-bit limbs, plus >= 60 guard bits): every precision that is a multiple of 128 bits up to 1536 bits,
+// plus the multiples of 64 bits the configured workloads use.
+#define B3D_FOR_EACH_L(X) X(4) X(6) X(10) X(12) X(14) X(18) X(22) X(26) X(30) X(34) X(38) X(42) X(46) X(50)
+
+#define B3D_CASE_L(Lv, ...) \
+  case Lv: {                \
+    constexpr int LL = Lv;  \
+    __VA_ARGS__;            \
+  } break;
+
+// Expands to a switch over the compiled limb counts; the statement may use the constexpr LL.
+#define B3D_DISPATCH_L(L, ...)                                                     \
+  switch (L) {                                                                    \
+    B3D_CASE_L(4, __VA_ARGS__)                                                           \
+    B3D_CASE_L(6, __VA_ARGS__)                                                           \
+    B3D_CASE_L(10, __VA_ARGS__)                                                          \
+    B3D_CASE_L(12, __VA_ARGS__)                                                          \
+    B3D_CASE_L(14, __VA_ARGS__)                                                          \
+    B3D_CASE_L(18, __VA_ARGS__)                                                          \
+    B3D_CASE_L(22, __VA_ARGS__)                                                          \
+    B3D_CASE_L(26, __VA_ARGS__)                                                          \
+    B3D_CASE_L(30, __VA_ARGS__)                                                          \
+    B3D_CASE_L(34, __VA_ARGS__)                                                          \
+    B3D_CASE_L(38, __VA_ARGS__)                                                          \
+    B3D_CASE_L(42, __VA_ARGS__)                                                          \
+    B3D_CASE_L(46, __VA_ARGS__)                                                          \
+    B3D_CASE_L(50, __VA_ARGS__)                                                          \
+    default:                                                                      \
+      b3d::set_error("unsupported limb count L=" + std::to_string(L) +            \
+                     " (compiled: 4 6 10 12 14 18 22 26 30 34 38 42 46 50)");     \
+      return 2;                                                                   \
+  }
+
+static inline unsigned b3d_blocks(int64_t n, int th) { return (unsigned)((n + th - 1) / th); }

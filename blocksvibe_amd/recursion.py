@@ -1,0 +1,145 @@
+"""Batched radial recursion of blocks_3d on the GPU (reference blocksvibe/recursion_gpu.py:379-514).
+
+For one operator (a Plan) the level loop runs, per level m, two libb3dhip kernels' worth of work:
+
+  Cauchy step  H(src, m, j', D') = h_inf[src][m][j'] + sum_i 2 / (D' - D_i) res[src][m - shift_i][j'][i]
+  pole step    res[p][m][j][P]   = Q(P, j) L_P H R_P^T        (S x S structure matrices)
+
+over all right-hand sides (four-point structures x lambda orders) at once: the batch B is the innermost
+dimension of every array.  Constants (M matrices, Q values, their pre-aligned forms) are built once per
+operator and reused by every run().
+"""
+import time
+from fractions import Fraction
+
+import numpy as np
+import torch
+
+from . import _lib, mpf
+from . import structure as S
+
+
+def number_pair(x, L):
+    """a number of any supported kind (decimal string, int, Fraction, mpmath mpf, (m, e) pair) ->
+    normalised (m, e), rounded half away from zero from its exact value"""
+    if isinstance(x, tuple):
+        return mpf.norm_int(int(x[0]), int(x[1]), L)
+    if isinstance(x, str):
+        x = Fraction(x)
+    if isinstance(x, int):
+        return mpf.norm_int(x, 0, L)
+    if isinstance(x, Fraction):
+        return mpf.norm_fraction(x.numerator, x.denominator, L)
+    return mpf.from_mpmath(x, L)
+
+
+class Recursion:
+    """Device constants and level tables of one Plan at L limbs.
+    M: dict {Mkey: matrix (rows of numbers)} for every key of plan.Mkeys (absent keys: zero matrices), an
+    MTable-like object with .pairs(key) -> rows of (m, e), or None for random O(1) matrices (synthetic
+    operators; the cost does not depend on the values)."""
+
+    def __init__(self, plan, M, L, seed=0):
+        if not _lib.lib().b3d_supported_limbs(L):
+            raise _lib.B3DError(f"no kernels compiled for L={L}")
+        if plan.dmax >= 1 << 15:
+            raise _lib.B3DError(f"|D' - D_i| up to {plan.dmax}: the Cauchy kernel needs < 2^15")
+        self.plan, self.L = plan, L
+        Sm = plan.Smax
+        nM = len(plan.Mkeys)
+        if M is None:
+            self.Mtab = mpf.random((max(nM, 1), Sm, Sm), L, spread=1, generator=np.random.default_rng(seed))
+        else:
+            vals = [(0, mpf.EZ)] * (max(nM, 1) * Sm * Sm)
+            for key, i in plan.Mkeys.items():
+                mat = M.get(key) if isinstance(M, dict) else M.pairs(key)
+                for a, row in enumerate(mat or []):
+                    for b, x in enumerate(row):
+                        vals[(i * Sm + a) * Sm + b] = number_pair(x, L)
+            self.Mtab = mpf.from_pairs(vals, L, (max(nM, 1), Sm, Sm))
+        qv = [Fraction(0)] * max(len(plan.Qkeys), 1)
+        for ((t, k), tj, sign), i in plan.Qkeys.items():
+            qv[i] = S.q_value(t, k, tj) * sign
+        self.Qtab = mpf.from_fractions(qv, L, (len(qv),))
+        self._constants()
+        self._upload_levels()
+
+    def _constants(self):
+        plan, Sm, L = self.plan, self.plan.Smax, self.L
+        self.Ralign_m, self.Ralign_e = mpf.align_rows(self.Mtab)            # (L, nM, S, S), (nM, S)
+        lq = torch.from_numpy(plan.LQpairs if len(plan.LQpairs) else np.zeros((1, 2), np.int64)).to(mpf.dev())
+        Lm = self.Mtab.take(lq[:, 0])                                       # (nLQ, S, S)
+        q = self.Qtab.take(lq[:, 1])                                        # (nLQ,)
+        qx = mpf.MPF(q.m[:, :, None, None].expand(L, len(lq), Sm, Sm).contiguous(),
+                     q.e[:, None, None].expand(len(lq), Sm, Sm).contiguous())
+        self.QL = mpf.mul(Lm, qx)
+        self.QLalign_m, self.QLalign_e = mpf.align_rows(self.QL)
+
+    def _upload_levels(self):
+        plan, d = self.plan, mpf.dev()
+        T = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a).astype(dt)).to(d)
+        O1, Jh = plan.order + 1, plan.Jh
+        self.levels = []
+        for lev in plan.levels:
+            if lev is None:
+                self.levels.append(None)
+                continue
+            self.levels.append({
+                "m": lev.m, "nH": lev.nH, "T": lev.T, "Kc": lev.Kc,
+                "col": T(lev.col.reshape(-1), np.int64), "cold": T(lev.cold.reshape(-1), np.int32),
+                "ncol": T(lev.ncol, np.int32), "rg": T(lev.rg, np.int32), "rd": T(lev.rd, np.int32),
+                "ghrow": T((lev.gsrc * O1 + lev.m) * Jh + lev.gtjp // 2, np.int64),
+                "t_hrow": T(lev.t_hrow, np.int64), "t_R": T(lev.t_R, np.int32), "t_qL": T(lev.t_qL, np.int32),
+                "t_slot": T(lev.t_slot, np.int64)})
+        live = [lv for lv in self.levels if lv is not None]
+        self.maxH = max((lv["nH"] for lv in live), default=1)
+        self.maxT = max((lv["T"] for lv in live), default=1)
+
+    def run(self, h_inf, hook=None, prof=None, res=None):
+        """h_inf: MPF (2, order + 1, Jh, Smax^2, B).  Returns the residue slot buffer MPF
+        (nphys + 1, Smax^2, B) (physical slots; the last one is the zero slot).  hook(m, res) runs after each
+        level (e.g. the derivative moments)."""
+        plan, L, Sm = self.plan, self.L, self.plan.Smax
+        S2 = Sm * Sm
+        assert h_inf.shape[:4] == (2, plan.order + 1, plan.Jh, S2), (h_inf.shape, plan.order, plan.Jh, S2)
+        B = h_inf.shape[-1]
+        N = S2 * B
+        h = h_inf.contiguous()
+        if res is None:
+            res = mpf.zeros((plan.nphys + 1, S2, B), L)
+        else:
+            res.m.zero_()
+            res.e.fill_(mpf.EZ)
+        H = mpf.zeros((self.maxH, S2, B), L)
+        Y = mpf.zeros((self.maxT, Sm, Sm, B), L)
+        st = mpf.stream_ptr()
+        P = lambda t: t.data_ptr()
+        for lv in self.levels:
+            if lv is None:
+                continue
+            t0 = time.time() if prof is not None else None
+            _lib.call("b3d_cauchy_level", L, P(res.m), P(res.e), res.numel(), N, P(h.m), P(h.e), h.numel(),
+                      P(lv["ghrow"]), P(lv["rg"]), P(lv["rd"]), lv["nH"], P(lv["col"]), P(lv["cold"]),
+                      P(lv["ncol"]), lv["Kc"], P(H.m), P(H.e), H.numel(), st)
+            if prof is not None:
+                torch.cuda.synchronize()
+                t1 = time.time()
+                prof["cauchy"] = prof.get("cauchy", 0.0) + t1 - t0
+            _lib.call("b3d_pole_level", L, Sm, B, lv["T"], P(H.m), P(H.e), H.numel(), P(lv["t_hrow"]),
+                      P(lv["t_R"]), P(lv["t_qL"]), P(lv["t_slot"]), P(self.Ralign_m), P(self.Ralign_e),
+                      self.Ralign_m[0].numel(), P(self.QLalign_m), P(self.QLalign_e), self.QLalign_m[0].numel(),
+                      P(Y.m), P(Y.e), Y.numel(), P(res.m), P(res.e), res.numel(), st)
+            if prof is not None:
+                torch.cuda.synchronize()
+                t2 = time.time()
+                prof["pole"] = prof.get("pole", 0.0) + t2 - t1
+            if hook is not None:
+                hook(lv["m"], res)
+                if prof is not None:
+                    torch.cuda.synchronize()
+                    prof["moments"] = prof.get("moments", 0.0) + time.time() - t2
+        return res
+
+    def slot(self, p, m, jidx, pidx):
+        """physical slot of residue (p, m, jidx, pidx) (meaningful after run() only without slot reuse)"""
+        return self.plan.phys_slot(p * self.plan.nslots + self.plan.offset(m, jidx) + pidx)

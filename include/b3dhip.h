@@ -1,0 +1,74 @@
+/* libb3dhip -- C-ABI of the MI355X (gfx950) blocks_3d hot path.
+ *
+ * Plain pointers and sizes only.  Device arrays are allocated by the caller (any HIP allocator); the
+ * library never allocates on these paths except where noted (b3d_rns_gemm takes a caller workspace).
+ * Every entry point returns 0 on success, nonzero on error (b3d_last_error() then describes it); launches
+ * are asynchronous on `stream` (a hipStream_t, NULL = default stream).
+ *
+ * Multiprecision arrays ("MPF"): mantissas as int32 limb planes -- limb k of entry i at m[k * ps + i],
+ * ps the plane stride, L limbs, two's complement, limb 0 least significant -- and exponents e[i] int64;
+ * value m * 2^e; normalised: bitlen(|m|) = 32 L - 2, zero = (0, EZ = -2^62).
+ * The reference's equivalent is the unvendored sdpvibe.mpf type reached from Python through DLPack
+ * (reference blocksvibe/cauchy.py:83-85, smallgemm.py:84-86, derivatives.py:478-482).
+ */
+#ifndef B3DHIP_H
+#define B3DHIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define B3D_ABI_VERSION 1
+
+/* Last error message of the calling thread (empty if none). */
+const char* b3d_last_error(void);
+int b3d_abi_version(void);
+/* 1 if kernels are compiled for L limbs (reference cli.py:270-274 gives L from the precision). */
+int b3d_supported_limbs(int L);
+
+/* ---- MPF primitives (replace sdpvibe.mpf.normalize / add / mul / _align_rows, called at reference
+ *      blocksvibe/cauchy.py:78,87, smallgemm.py:77-79,88, recursion_gpu.py:432-436,457,460,496-509) ---- */
+
+/* Wide two's complement integers W (LW <= 2L+4 limb planes, plane stride wps) times 2^ein[i] ->
+ * normalised (round half away from zero). */
+int b3d_mpf_norm_wide(int L, int LW, const int32_t* W, int64_t wps, const int64_t* ein, int32_t* om,
+                      int64_t* oe, int64_t ops, int64_t n, void* stream);
+/* c[i] = a[i*sa] + b[i*sb]  (sa, sb in {0, 1}: broadcast or elementwise) */
+int b3d_mpf_add(int L, const int32_t* am, const int64_t* ae, int64_t aps, int64_t sa, const int32_t* bm,
+                const int64_t* be, int64_t bps, int64_t sb, int32_t* cm, int64_t* ce, int64_t cps, int64_t n,
+                void* stream);
+/* c[i] = a[i*sa] * b[i*sb] */
+int b3d_mpf_mul(int L, const int32_t* am, const int64_t* ae, int64_t aps, int64_t sa, const int32_t* bm,
+                const int64_t* be, int64_t bps, int64_t sb, int32_t* cm, int64_t* ce, int64_t cps, int64_t n,
+                void* stream);
+/* Align each row of a (rows, K) array (entry (r,k) at r*rs + k*ks) to its largest exponent: aligned
+ * mantissas to om (contiguous (rows, K), plane stride ops), row exponents to oe[rows]. */
+int b3d_mpf_align_rows(int L, const int32_t* m, const int64_t* e, int64_t ps, int64_t rows, int64_t K,
+                       int64_t rs, int64_t ks, int32_t* om, int64_t ops, int64_t* oe, void* stream);
+
+/* ---- radial recursion, one level (replaces the body of reference Recursion.run,
+ *      blocksvibe/recursion_gpu.py:446-476: cauchy_kernel cauchy.py:73-87, _pole :483-492) ---- */
+
+/* Cauchy step.  Slot buffer rm/re (plane stride rps, N = S*S*B columns per slot); h_inf hm/he (plane
+ * stride hps, N columns per row); groups g < nb: ghrow[g] = h_inf row, col[g*Kc + i] physical slots of
+ * the prime poles (i < ncol[g]), cold[g*Kc + i] their doubled positions D_i; H rows h < nH: rg[h] group,
+ * rd[h] doubled D'.  Output H (nH, N) at om/oe (plane stride ops).  Requires |D' - D_i| < 2^15. */
+int b3d_cauchy_level(int L, const int32_t* rm, const int64_t* re, int64_t rps, int64_t N, const int32_t* hm,
+                     const int64_t* he, int64_t hps, const int64_t* ghrow, const int32_t* rg, const int32_t* rd,
+                     int64_t nH, const int64_t* col, const int32_t* cold, const int32_t* ncol, int Kc, int32_t* om,
+                     int64_t* oe, int64_t ops, void* stream);
+
+/* Pole step for T tasks: Y = Ralign[tR[t]] . H[thrw[t]]^T, then res[tslot[t]] = QLalign[tQL[t]] . Y^T,
+ * S x S structure matrices, B right-hand sides.  Ralign/QLalign: (L, ntab, S, S) row-aligned mantissas
+ * (plane strides rtps/qps) with row exponents (ntab, S).  Y: scratch (L, T, S, S, B), plane stride yps. */
+int b3d_pole_level(int L, int S, int64_t B, int64_t T, const int32_t* Hm, const int64_t* He, int64_t hps,
+                   const int64_t* thrw, const int32_t* tR, const int32_t* tQL, const int64_t* tslot,
+                   const int32_t* Rm, const int64_t* Re, int64_t rtps, const int32_t* Qm, const int64_t* Qe,
+                   int64_t qps, int32_t* Ym, int64_t* Ye, int64_t yps, int32_t* rm, int64_t* re, int64_t rps,
+                   void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,18 @@
+"""CPU oracle of the blocks_3d hot path -- TEST INFRASTRUCTURE ONLY.
+
+Nothing in the product (`blocksvibe_amd`) imports this package.  Only `tests/`, `__graft_entry__.smoke()`
+and the `cpu_baseline` leg of `bench.py` use it, and only as the checker.
+
+Modules:
+  mpf_ref        bit-level specification of the multiprecision number format and of every rounding the
+                 HIP kernels perform, restated with Python integers (pinned by exact-rational error bounds,
+                 brute force over small inputs and special cases: tests/test_oracle_mpf.py)
+  structure_ref  pole/spin bookkeeping (reference blocksvibe/structure.py:16-83), scalar form
+  recursion_ref  the radial recursion in exact arithmetic (reference blocksvibe/recursion_ref.py:11-44)
+  plan_ref       per-task index plan (reference blocksvibe/plan_ref.py:9-132)
+  recursion_bits the GPU recursion's exact bit-level semantics (Cauchy and pole steps) on top of mpf_ref
+  derivs_ref     residues -> numerators, mpmath (reference tests/test_blocks_derivatives.py:13-80)
+  output_ref     write_output maps, mpmath (reference blocksvibe/output.py:35-95, tests/test_blocks_output.py:13-47)
+  json_ref       decimal formatting of exact binary numbers (reference blocksvibe/b3djson.py:20-90)
+  c/             a C restatement of the recursion kernels (CPU baseline and large-size bit parity)
+"""
