@@ -26,7 +26,7 @@ def declarations():
     txt = open(HEADER).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     out = []
-    for m in re.finditer(r"^\s*(const char\*|int|void)\s+(b3d_\w+)\s*\(([^)]*)\)\s*;", txt, flags=re.M):
+    for m in re.finditer(r"^\s*(const char\*|int64_t|int|void)\s+(b3d_\w+)\s*\(([^)]*)\)\s*;", txt, flags=re.M):
         ret, name, args = m.group(1), m.group(2), m.group(3).strip()
         types = []
         if args and args != "void":
@@ -53,7 +53,7 @@ def lib():
             raise B3DError(f"cannot load {LIBPATH}: {e}")
         for name, ret, types in declarations():
             f = getattr(L, name)
-            f.restype = ctypes.c_char_p if ret == "const char*" else (None if ret == "void" else I32)
+            f.restype = {"const char*": ctypes.c_char_p, "void": None, "int64_t": I64}.get(ret, I32)
             f.argtypes = types
         _lib = L
     return _lib

@@ -1,0 +1,285 @@
+// libb3dhip: exact multiprecision GEMM through a residue number system on int8 MFMA.
+//
+// C[b] = A[b] B[b] with A aligned per row and B per column (one common exponent per line), the aligned
+// mantissa products summed exactly, one rounding per entry.  This replaces the unvendored sdpvibe.mpf.gemm
+// of the reference (called at blocksvibe/recursion_gpu.py:53-79, derivatives.py:540-580,
+// output.py:189,211).
+//
+// Residues: nmod primes p < 2^15 whose product M exceeds 2^(2 NB + log2 K + 4), so that the signed result
+// is recovered exactly from its residues.  A residue r (balanced, |r| < 2^14) is split into two signed
+// int8 digits r = 256 h + l (h in [-64, 64], l in [-128, 127]); per prime, four int8 GEMMs on
+// v_mfma_i32_16x16x64_i8 (int32 accumulation, exact for K < 2^17) give
+//   r_A r_B summed = 2^16 (h h) + 2^8 (h l + l h) + (l l).
+// Stages: (1) per-line exponent maxima, (2) aligned limbs -> residue digits (int8 planes, K contiguous,
+// padded to 64), (3) MFMA products reduced mod p -> uint16 residues, (4) CRT lift (y_t = c_t (M/p_t)^-1,
+// C = sum_t y_t M/p_t - round(sum_t y_t / p_t) M, exact in 2L + 2 limbs) and normalisation.
+#include "common.hpp"
+#include "mp.hpp"
+#include "../../include/b3dhip.h"
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+static inline unsigned grid_for(int64_t n, int th) {
+  int64_t b = (n + th - 1) / th;
+  if (b > (1 << 22)) b = 1 << 22;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+// x mod p for 0 <= x < 2^53, through a double reciprocal (one correction either way)
+__device__ __forceinline__ uint32_t mod_small(uint64_t x, uint32_t p, double rp) {
+  const double q = floor((double)x * rp);
+  int64_t r = (int64_t)x - (int64_t)q * (int64_t)p;
+  if (r < 0) r += p;
+  if (r >= (int64_t)p) r -= p;
+  return (uint32_t)r;
+}
+
+// signed v (|v| < 2^52) mod p in [0, p)
+__device__ __forceinline__ uint32_t mod_signed(int64_t v, uint32_t p, double rp) {
+  const double q = floor((double)v * rp);
+  int64_t r = v - (int64_t)q * (int64_t)p;
+  if (r < 0) r += p;
+  if (r >= (int64_t)p) r -= p;
+  return (uint32_t)r;
+}
+
+// (1) exponent maximum per line: entry (b, line, k) at b*sb + line*sl + k*sk; Eline[b * lines + line]
+__global__ __launch_bounds__(256) void k_line_max(const int32_t* __restrict__ m, const i64* __restrict__ e, i64 top,
+                                                  i64 nb, i64 lines, i64 K, i64 sb, i64 sl, i64 sk,
+                                                  i64* __restrict__ Eline) {
+  for (i64 t = blockIdx.x * (i64)blockDim.x + threadIdx.x; t < nb * lines; t += (i64)gridDim.x * blockDim.x) {
+    const i64 b = t / lines, line = t % lines;
+    i64 E = B3D_EZ;
+    const i64 base = b * sb + line * sl;
+    for (i64 k = 0; k < K; ++k) {
+      const i64 idx = base + k * sk;
+      if (m[top + idx] != 0) {
+        const i64 x = e[idx];
+        E = x > E ? x : E;
+      }
+    }
+    Eline[t] = E;
+  }
+}
+
+// (2) residue digits.  Output planes out[(t*2 + d) * plane + (b * Lp + line) * Kp + k], d = 0 high digit,
+// 1 low digit, zero in the padding (line >= lines or k >= K).
+template <int L>
+__global__ __launch_bounds__(256) void k_to_residues(const int32_t* __restrict__ m, const i64* __restrict__ e, i64 ps,
+                                                     i64 nb, i64 lines, i64 K, i64 sb, i64 sl, i64 sk,
+                                                     const i64* __restrict__ Eline, i64 estride, int Lp, int Kp, int nmod,
+                                                     const uint16_t* __restrict__ primes,
+                                                     const uint32_t* __restrict__ pw, const double* __restrict__ rp,
+                                                     int8_t* __restrict__ out, i64 plane) {
+  const i64 total = nb * (i64)Lp * Kp;
+  for (i64 t = blockIdx.x * (i64)blockDim.x + threadIdx.x; t < total; t += (i64)gridDim.x * blockDim.x) {
+    const i64 k = t % Kp, line = (t / Kp) % Lp, b = t / ((i64)Kp * Lp);
+    u32 x[L];
+    bool ng = false;
+    bool nz = false;
+    if (line < lines && k < K) {
+      const i64 idx = b * sb + line * sl + k * sk;
+      mp::load(x, m, ps, idx);
+      if (x[L - 1] != 0u) {
+        nz = true;
+        ng = mp::to_mag(x);
+        const i64 sh = Eline[b * estride + line] - e[idx];
+        mp::shr(x, sh > 32 * L ? 32 * L : (int)sh);
+      }
+    }
+    for (int tm = 0; tm < nmod; ++tm) {
+      int h = 0, l = 0;
+      if (nz) {
+        const uint32_t p = primes[tm];
+        const uint32_t* pwt = pw + (i64)tm * L;
+        uint64_t s = 0;
+#pragma unroll
+        for (int q = 0; q < L; ++q) s += (uint64_t)x[q] * pwt[q];  // < L 2^47 < 2^53
+        uint32_t r = mod_small(s, p, rp[tm]);
+        if (ng && r) r = p - r;
+        int rb = (int)r;
+        if (rb > (int)(p >> 1)) rb -= (int)p;  // balanced, |rb| < 2^14
+        h = (rb + 128) >> 8;
+        l = rb - 256 * h;
+      }
+      out[(i64)(tm * 2 + 0) * plane + t] = (int8_t)h;
+      out[(i64)(tm * 2 + 1) * plane + t] = (int8_t)l;
+    }
+  }
+}
+
+// (3) per-prime products on MFMA: each wave one 16 x 16 output tile, a block of 4 waves covers 16 x 64.
+// Residues out: Cr[(t * nb + b) * M * Nc + i * Nc + j] (uint16).
+__global__ __launch_bounds__(256) void k_rns_mma(const int8_t* __restrict__ A8, i64 aplane, int a_shared,
+                                                 const int8_t* __restrict__ B8, i64 bplane, int Mp, int Np, int Kp,
+                                                 int M, int N, const uint16_t* __restrict__ primes,
+                                                 const double* __restrict__ rp, uint16_t* __restrict__ Cr, i64 nb) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int tm = blockIdx.y;
+  const i64 b = blockIdx.z;
+  const int ntn = (Np + 63) / 64;
+  const int tile_m = blockIdx.x / ntn, tile_n = blockIdx.x % ntn;
+  const int row0 = tile_m * 16, col0 = tile_n * 64 + wave * 16;
+  if (col0 >= Np) return;
+  const i64 ab = a_shared ? 0 : b;
+  const int8_t* Ah = A8 + (i64)(tm * 2 + 0) * aplane + (ab * Mp) * Kp;
+  const int8_t* Al = A8 + (i64)(tm * 2 + 1) * aplane + (ab * Mp) * Kp;
+  const int8_t* Bh = B8 + (i64)(tm * 2 + 0) * bplane + (b * Np) * Kp;
+  const int8_t* Bl = B8 + (i64)(tm * 2 + 1) * bplane + (b * Np) * Kp;
+  const int r = lane & 15, g = lane >> 4;
+  v4i hh = {0, 0, 0, 0}, mid = {0, 0, 0, 0}, ll = {0, 0, 0, 0};
+  for (int k0 = 0; k0 < Kp; k0 += 64) {
+    const i64 ao = (i64)(row0 + r) * Kp + k0 + g * 16;
+    const i64 bo = (i64)(col0 + r) * Kp + k0 + g * 16;
+    const v4i ah = *(const v4i*)(Ah + ao);
+    const v4i al = *(const v4i*)(Al + ao);
+    const v4i bh = *(const v4i*)(Bh + bo);
+    const v4i bl = *(const v4i*)(Bl + bo);
+    hh = __builtin_amdgcn_mfma_i32_16x16x64_i8(ah, bh, hh, 0, 0, 0);
+    mid = __builtin_amdgcn_mfma_i32_16x16x64_i8(ah, bl, mid, 0, 0, 0);
+    mid = __builtin_amdgcn_mfma_i32_16x16x64_i8(al, bh, mid, 0, 0, 0);
+    ll = __builtin_amdgcn_mfma_i32_16x16x64_i8(al, bl, ll, 0, 0, 0);
+  }
+  const uint32_t p = primes[tm];
+  const double rpt = rp[tm];
+  const int col = col0 + r;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int row = row0 + 4 * g + i;
+    if (row < M && col < N) {
+      const int64_t v = (int64_t)hh[i] * 65536 + (int64_t)mid[i] * 256 + (int64_t)ll[i];
+      Cr[((i64)tm * nb + b) * M * N + (i64)row * N + col] = (uint16_t)mod_signed(v, p, rpt);
+    }
+  }
+}
+
+// (4) CRT lift and normalisation; output entry (b, i, j) at b*csb + i*csi + j*csj (+ column offset j0)
+template <int L>
+__global__ __launch_bounds__(256) void k_crt(const uint16_t* __restrict__ Cr, i64 nb, int M, int N, int nmod,
+                                             const uint16_t* __restrict__ primes, const uint16_t* __restrict__ inv,
+                                             const double* __restrict__ rp, const uint32_t* __restrict__ Mp,
+                                             const uint32_t* __restrict__ Mfull, const i64* __restrict__ EA,
+                                             int a_shared, const i64* __restrict__ EB, i64 j0, i64 Nfull,
+                                             int32_t* __restrict__ cm, i64* __restrict__ ce, i64 cps, i64 csb,
+                                             i64 csi, i64 csj) {
+  constexpr int LW = 2 * L + 2;
+  const i64 total = nb * (i64)M * N;
+  for (i64 t = blockIdx.x * (i64)blockDim.x + threadIdx.x; t < total; t += (i64)gridDim.x * blockDim.x) {
+    const i64 j = t % N, i = (t / N) % M, b = t / ((i64)N * M);
+    const i64 ea = EA[(a_shared ? 0 : b) * M + i];
+    const i64 eb = EB[b * Nfull + j0 + j];
+    u32 out[L];
+    i64 eo = B3D_EZ;
+    if (ea == B3D_EZ || eb == B3D_EZ) {
+      mp::zero(out);
+    } else {
+      u32 acc[LW];
+      mp::zero(acc);
+      double qs = 0.0;
+      for (int tm = 0; tm < nmod; ++tm) {
+        const uint32_t p = primes[tm];
+        const uint32_t c = Cr[((i64)tm * nb + b) * M * N + i * (i64)N + j];
+        const uint32_t y = mod_small((uint64_t)c * inv[tm], p, rp[tm]);
+        qs += (double)y * rp[tm];
+        const uint32_t* mp_t = Mp + (i64)tm * LW;
+        u64 cy = 0;
+#pragma unroll
+        for (int q = 0; q < LW; ++q) {
+          const u64 v = (u64)y * mp_t[q] + acc[q] + cy;
+          acc[q] = (u32)v;
+          cy = v >> 32;
+        }
+      }
+      const u32 qq = (u32)rint(qs);
+      // acc -= qq * M
+      u64 cy = 0;
+      u32 bor = 0;
+#pragma unroll
+      for (int q = 0; q < LW; ++q) {
+        const u64 prod = (u64)qq * Mfull[q] + cy;
+        cy = prod >> 32;
+        const u64 v = (u64)acc[q] - (u64)(u32)prod - (u64)bor;
+        acc[q] = (u32)v;
+        bor = (u32)(v >> 63);
+      }
+      eo = mp::norm<L, LW>(acc, ea + eb, out);
+    }
+    const i64 o = b * csb + i * csi + (j0 + j) * csj;
+    mp::store(out, cm, cps, o);
+    ce[o] = eo;
+  }
+}
+
+extern "C" {
+
+int64_t b3d_rns_workspace_bytes(int nmod, int64_t nbA, int64_t nb, int64_t M, int64_t Nc, int64_t K) {
+  const int64_t Mp = (M + 15) / 16 * 16, Np = (Nc + 63) / 64 * 64, Kp = (K + 63) / 64 * 64;
+  int64_t bytes = 0;
+  bytes += 2 * (int64_t)nmod * nbA * Mp * Kp;  // A digits
+  bytes += 2 * (int64_t)nmod * nb * Np * Kp;   // B digits (one chunk)
+  bytes += 2 * (int64_t)nmod * nb * M * Nc;    // residues of C (uint16)
+  bytes += 8 * (nbA * M + nb * (int64_t)0);    // row exponents (column exponents are separate)
+  return bytes + 1024;
+}
+
+int b3d_rns_gemm(int L, int nmod, const uint16_t* primes, const uint32_t* pw, const uint16_t* inv, const double* rp,
+                 const uint32_t* Mp, const uint32_t* Mfull, int64_t nb, int64_t M, int64_t N, int64_t K, int a_shared,
+                 const int32_t* Am, const int64_t* Ae, int64_t aps, int64_t asb, int64_t asi, int64_t ask,
+                 const int32_t* Bm, const int64_t* Be, int64_t bps, int64_t bsb, int64_t bsk, int64_t bsj, int32_t* Cm,
+                 int64_t* Ce, int64_t cps, int64_t csb, int64_t csi, int64_t csj, int64_t Nc, int64_t* EB,
+                 void* workspace, int64_t ws_bytes, void* stream) {
+  if (nb <= 0 || M <= 0 || N <= 0) return 0;
+  if (K <= 0 || K > (1 << 16)) {
+    b3d::set_error("b3d_rns_gemm: need 0 < K <= 2^16");
+    return 2;
+  }
+  if (nb > 65535) {
+    b3d::set_error("b3d_rns_gemm: batch must be < 65536");
+    return 2;
+  }
+  if (Nc <= 0 || Nc > N) Nc = N;
+  const int64_t nbA = a_shared ? 1 : nb;
+  if (b3d_rns_workspace_bytes(nmod, nbA, nb, M, Nc, K) > ws_bytes) {
+    b3d::set_error("b3d_rns_gemm: workspace too small for the column chunk");
+    return 2;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const int Mpad = (int)((M + 15) / 16 * 16), Npad = (int)((Nc + 63) / 64 * 64), Kpad = (int)((K + 63) / 64 * 64);
+  char* w = (char*)workspace;
+  int8_t* A8 = (int8_t*)w;
+  w += 2 * (int64_t)nmod * nbA * Mpad * Kpad;
+  int8_t* B8 = (int8_t*)w;
+  w += 2 * (int64_t)nmod * nb * Npad * Kpad;
+  uint16_t* Cr = (uint16_t*)w;
+  w += 2 * (int64_t)nmod * nb * M * Nc;
+  w = (char*)(((uintptr_t)w + 255) & ~(uintptr_t)255);
+  int64_t* EA = (int64_t*)w;
+  const int64_t aplane = nbA * Mpad * Kpad, bplane = nb * (int64_t)Npad * Kpad;
+  // A: rows
+  k_line_max<<<grid_for(nbA * M, 256), 256, 0, s>>>(Am, Ae, (int64_t)(L - 1) * aps, nbA, M, K, asb, asi, ask, EA);
+  B3D_DISPATCH_L(L, k_to_residues<LL><<<grid_for(nbA * (int64_t)Mpad * Kpad, 256), 256, 0, s>>>(
+                        Am, Ae, aps, nbA, M, K, asb, asi, ask, EA, M, Mpad, Kpad, nmod, primes, pw, rp, A8, aplane));
+  // B: columns (all of them once: exponents over k per column)
+  k_line_max<<<grid_for(nb * N, 256), 256, 0, s>>>(Bm, Be, (int64_t)(L - 1) * bps, nb, N, K, bsb, bsj, bsk, EB);
+  if (b3d::check_launch("b3d_rns_gemm (prep)")) return 1;
+  for (int64_t j0 = 0; j0 < N; j0 += Nc) {
+    const int64_t nc = (N - j0) < Nc ? (N - j0) : Nc;
+    const int Np = (int)((nc + 63) / 64 * 64);
+    // column chunk [j0, j0 + nc): shift the B base pointer, column exponents indexed with the full N
+    B3D_DISPATCH_L(L, k_to_residues<LL><<<grid_for(nb * (int64_t)Np * Kpad, 256), 256, 0, s>>>(
+                          Bm + j0 * bsj, Be + j0 * bsj, bps, nb, nc, K, bsb, bsj, bsk, EB + j0, N, Np, Kpad, nmod,
+                          primes, pw, rp, B8, nb * (int64_t)Np * Kpad));
+    dim3 grid((unsigned)(((Mpad + 15) / 16) * ((Np + 63) / 64)), (unsigned)nmod, (unsigned)nb);
+    k_rns_mma<<<grid, 256, 0, s>>>(A8, aplane, a_shared, B8, nb * (int64_t)Np * Kpad, Mpad, Np, Kpad, (int)M,
+                                   (int)nc, primes, rp, Cr, nb);
+    B3D_DISPATCH_L(L, k_crt<LL><<<grid_for(nb * M * nc, 256), 256, 0, s>>>(
+                          Cr, nb, (int)M, (int)nc, nmod, primes, inv, rp, Mp, Mfull, EA, a_shared, EB, j0, N, Cm, Ce,
+                          cps, csb, csi, csj));
+    if (b3d::check_launch("b3d_rns_gemm (chunk)")) return 1;
+  }
+  (void)bplane;
+  return 0;
+}
+
+}  // extern "C"

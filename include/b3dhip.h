@@ -68,6 +68,42 @@ int b3d_pole_level(int L, int S, int64_t B, int64_t T, const int32_t* Hm, const 
                    int64_t qps, int32_t* Ym, int64_t* Ye, int64_t yps, int32_t* rm, int64_t* re, int64_t rps,
                    void* stream);
 
+/* ---- derivative moments during the recursion (replace the momacc/momflush kernels and the staging of
+ *      reference blocksvibe/derivatives.py:46-189, 474-510) ---- */
+
+/* Staging after a level: for t < T, column c < N: stage[woff + row[t]*N + c] = res[slot[t]*N + c] *
+ * pw[nn[t]] (pw: table of r_c^n, plane stride pps). */
+int b3d_moments_stage(int L, const int32_t* rm, const int64_t* re, int64_t rps, const int64_t* slot,
+                      const int64_t* row, const int32_t* nn, int64_t T, int64_t N, const int32_t* pm,
+                      const int64_t* pe, int64_t pps, int32_t* sm, int64_t* se, int64_t sps, int64_t woff,
+                      void* stream);
+/* Flush W staged levels m0..m0+W-1 into the moments v (L, nrows, N, R1): v[row, c, k] += sum_w
+ * stage[w, row, c] * n_w^k, n_w = m0 + w + srow[row]; NK: exact n^k as KL uint32 limb planes (plane
+ * stride nkps, entry n*R1 + k), NKb their bit lengths.  KL <= 8. */
+int b3d_moments_flush(int L, int KL, const int32_t* sm, const int64_t* se, int64_t sps, int W, int m0,
+                      const int32_t* srow, const uint32_t* NK, int64_t nkps, const int32_t* NKb, int32_t* vm,
+                      int64_t* ve, int64_t vps, int64_t nrows, int64_t N, int R1, void* stream);
+
+/* ---- exact multiprecision GEMM through residues on int8 MFMA (replaces the unvendored sdpvibe.mpf.gemm
+ *      reached through reference blocksvibe/recursion_gpu.py:53-79 gemm/bgemm, called at
+ *      derivatives.py:540,554,560,567,580 and output.py:189,211) ---- */
+
+/* Workspace for one column chunk of Nc columns: nbA = 1 if A is shared by the batch, else nb. */
+int64_t b3d_rns_workspace_bytes(int nmod, int64_t nbA, int64_t nb, int64_t M, int64_t Nc, int64_t K);
+
+/* C[b] = A[b] B[b] for b < nb: A entry (b,i,k) at b*asb + i*asi + k*ask (a_shared: one A for all b), B entry
+ * (b,k,j) at b*bsb + k*bsk + j*bsj, C entry (b,i,j) at b*csb + i*csi + j*csj.  A aligned per row, B per
+ * column, exact sum of products, one rounding.  Residue tables (device): nmod primes < 2^15, pw[t*L + l] =
+ * 2^(32 l) mod p_t, inv[t] = (M/p_t)^-1 mod p_t, rp[t] = 1/p_t, Mp[t*(2L+2) + q] limbs of M/p_t, Mfull the
+ * 2L+2 limbs of M = prod p_t (>= 2^(2(32L-2) + log2 K + 4)).  EB: scratch of nb*N int64.  Columns are
+ * processed Nc at a time within `workspace` (b3d_rns_workspace_bytes).  Requires K <= 2^16, nb < 65536. */
+int b3d_rns_gemm(int L, int nmod, const uint16_t* primes, const uint32_t* pw, const uint16_t* inv, const double* rp,
+                 const uint32_t* Mp, const uint32_t* Mfull, int64_t nb, int64_t M, int64_t N, int64_t K, int a_shared,
+                 const int32_t* Am, const int64_t* Ae, int64_t aps, int64_t asb, int64_t asi, int64_t ask,
+                 const int32_t* Bm, const int64_t* Be, int64_t bps, int64_t bsb, int64_t bsk, int64_t bsj, int32_t* Cm,
+                 int64_t* Ce, int64_t cps, int64_t csb, int64_t csi, int64_t csj, int64_t Nc, int64_t* EB,
+                 void* workspace, int64_t ws_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,55 @@
+"""Residue-number-system GEMM on int8 MFMA (blocksvibe_amd.rns) bit for bit against the CPU big-integer
+product of the aligned mantissas (oracle.mpf_ref.gemm): batched and shared-A forms, K not a multiple of
+the MFMA depth, zero rows/columns, wide exponent spreads, several column chunks, L from 4 to 50."""
+import random
+
+import pytest
+
+from oracle import mpf_ref as R
+
+pytestmark = pytest.mark.gpu
+
+
+def rand_pair(rng, L, spread):
+    if rng.random() < 0.05:
+        return 0, R.EZ
+    nb = 32 * L - 2
+    m = rng.getrandbits(nb - 1) | (1 << (nb - 1))
+    if rng.random() < 0.05:
+        m = (1 << nb) - 1
+    return (-m if rng.random() < 0.5 else m), rng.randint(-spread, spread) - nb
+
+
+def mat(rng, L, r, c, spread):
+    return [[rand_pair(rng, L, spread) for _ in range(c)] for _ in range(r)]
+
+
+@pytest.mark.parametrize("L,nb,M,K,N,shared,budget", [
+    (4, 3, 5, 70, 100, False, None),
+    (10, 2, 17, 1, 33, True, None),
+    (10, 4, 3, 130, 200, False, 1 << 20),       # several column chunks
+    (34, 2, 12, 61, 150, True, None),
+    (34, 3, 6, 136, 80, False, None),
+    (50, 1, 4, 40, 70, False, None),
+])
+def test_rns_gemm_bit_exact(L, nb, M, K, N, shared, budget):
+    from blocksvibe_amd import mpf, rns
+    rng = random.Random(L * 1000 + K)
+    spread = 60
+    As = [mat(rng, L, M, K, spread)] if shared else [mat(rng, L, M, K, spread) for _ in range(nb)]
+    Bs = [mat(rng, L, K, N, spread) for _ in range(nb)]
+    Bs[0][min(2, K - 1)] = [(0, R.EZ)] * N                     # a zero row of B
+    for k in range(K):                                          # a zero column of B
+        Bs[-1][k][N // 2] = (0, R.EZ)
+    As[0][M - 1] = [(0, R.EZ)] * K                              # a zero row of A
+    flatA = [x for a in As for row in a for x in row]
+    flatB = [x for b in Bs for row in b for x in row]
+    A = mpf.from_pairs(flatA, L, (M, K) if shared else (nb, M, K))
+    B = mpf.from_pairs(flatB, L, (nb, K, N))
+    print(f"L={L} nb={nb} M={M} K={K} N={N} shared={shared} nmod={rns.n_moduli(L, K)}", flush=True)
+    C = rns.gemm(A, B, budget=budget).to_host()
+    for b in range(nb):
+        ref = R.gemm(As[0] if shared else As[b], Bs[b], L)
+        for i in range(M):
+            for j in range(N):
+                assert C[(b * M + i) * N + j] == ref[i][j], (b, i, j)
