@@ -1,0 +1,197 @@
+"""Throughput of the batched blocks_3d GPU chain on one MI355X.
+
+Workload (the nmax=6 spin-2 external batch): four spin-2 externals, every operator group
+(2j12, 2j43) in {0,2,4,6,8}^2 (25 groups), 22 four-point structures per group (11 with sign +1, 11 with
+sign -1), internal spins 0..30, order 60, Lambda 11 (6 lambda orders per task), kept pole order 14,
+1024 bits (34 limbs), xt coordinates, Delta_1+2 = 6.  550 tasks x 31 spins = 17,050 blocks (one block =
+one (task, internal spin) table = one blocks_3d JSON file).  Operators (M matrices) and h_inf are seeded
+synthetic data of the real shapes (the cost of every stage does not depend on the values).
+
+One step = the whole chain for every task of the workload, inputs resident in HBM: radial recursion
+(Cauchy + pole steps), derivative moments, numerator GEMMs, write_output GEMM.  Per-operator constants
+(plan, M tables, numerator and output maps) are built before the timed region.  The JSON text formatting
+(host CPU) is not part of a step.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--groups all|heavy] [--plain]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+ORDER, LAMBDA, KEPT, PREC, TJMAX, D12 = 60, 11, 14, 1024, 60, "6"
+TWO_JS = (4, 4, 4, 4)
+NQ_PER_SIGN = 11
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--groups", default="all", help="all | heavy (2j12 = 2j43 = 8 only) | comma list a:b")
+    ap.add_argument("--plain", action="store_true", help="also time the plain per-task path on one group")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
+    ap.add_argument("--json-out", default=None)
+    return ap.parse_args()
+
+
+def group_list(spec):
+    if spec == "all":
+        return [(a, b) for a in (0, 2, 4, 6, 8) for b in (0, 2, 4, 6, 8)]
+    if spec == "heavy":
+        return [(8, 8)]
+    return [tuple(int(x) for x in g.split(":")) for g in spec.split(",")]
+
+
+class GroupRun:
+    """one operator group: constants and resident inputs for both signs"""
+
+    def __init__(self, tj12, tj43, L, seed):
+        import torch
+        from blocksvibe_amd import mpf
+        from blocksvibe_amd.derivatives import Derivs
+        from blocksvibe_amd.output import Output
+        from blocksvibe_amd.plan import Plan
+        from blocksvibe_amd.recursion import Recursion
+        self.tj12, self.tj43, self.L = tj12, tj43, L
+        self.plan = Plan(tj12, tj43, TJMAX, ORDER, ((0, 0), (1, 1)), reuse=True)
+        self.rec = Recursion(self.plan, None, L, seed=seed)
+        self.subs = []
+        Sm, Jh = self.plan.Smax, self.plan.Jh
+        for sign in (1, -1):
+            p0 = (1 - sign) // 2
+            der = Derivs(self.plan, LAMBDA - p0, L, kept_pole_order=KEPT)
+            out = Output("xt", LAMBDA, p0, D12, L)
+            ns = list(range(p0, LAMBDA + 1, 2))
+            hs = [mpf.random_device((2, ORDER + 1, Jh, Sm * Sm, NQ_PER_SIGN), L, seed=seed * 100 + 10 * p0 + i)
+                  for i, n in enumerate(ns)]
+            self.subs.append((sign, p0, der, out, ns, hs))
+        torch.cuda.synchronize()
+
+    def blocks(self):
+        return 2 * NQ_PER_SIGN * (TJMAX // 2 + 1)
+
+    def run(self, prof=None, keep=False):
+        """batched path: per sign, per lambda order one recursion over all structures"""
+        results = []
+        for sign, p0, der, out, ns, hs in self.subs:
+            nums = []
+            for n, h in zip(ns, hs):
+                dn = der.for_order(LAMBDA - n)
+                dn.start(h.shape[-1])
+                self.rec.run(h, hook=dn.accumulate, prof=prof)
+                nums.append(dn.finish(h))
+            r = out.apply_per_order(nums)
+            if keep:
+                results.append(r)
+        return results
+
+    def run_plain(self, task_limit=None):
+        """plain per-task path: one structure at a time (B = 1), no batching"""
+        count = 0
+        for sign, p0, der, out, ns, hs in self.subs:
+            for q in range(NQ_PER_SIGN):
+                if task_limit is not None and count >= task_limit:
+                    return count
+                nums = []
+                for n, h in zip(ns, hs):
+                    dn = der.for_order(LAMBDA - n)
+                    hq = h[:, :, :, :, q:q + 1].contiguous()
+                    dn.start(1)
+                    self.rec.run(hq, hook=dn.accumulate)
+                    nums.append(dn.finish(hq))
+                out.apply_per_order(nums)
+                count += 1
+        return count
+
+
+def cpu_baseline(runs, seconds=12.0):
+    """the oracle's bit-level restatement (oracle/mpf_ref: Cauchy terms and pole-step entries at 34 limbs,
+    pure Python on one core), timed on a bounded sample and scaled by the workload's term counts"""
+    import random
+    from oracle import mpf_ref as R
+    L = 34
+    rng = random.Random(1)
+    nb = R.NB(L)
+    rp = lambda: ((rng.getrandbits(nb - 1) | (1 << (nb - 1))) * rng.choice((-1, 1)), rng.randint(-4, 4) - nb)
+    # Cauchy: groups of 60 columns
+    xs = [rp() for _ in range(60)]
+    ds = [rng.randint(1, 300) * rng.choice((-1, 1)) for _ in range(60)]
+    t0, nterm = time.time(), 0
+    while time.time() - t0 < seconds / 2:
+        R.cauchy(xs, ds, L)
+        nterm += 60
+    t_term = (time.time() - t0) / nterm
+    # pole step entry: 5-term dot products, two per residue entry
+    a = [rp() for _ in range(5)]
+    am, ae = R.align(a)
+    t0, nent = time.time(), 0
+    while time.time() - t0 < seconds / 2:
+        xm, xe = R.align([rp() for _ in range(5)])
+        R.gemm_aligned(am, ae, xm, xe, L)
+        nent += 1
+    t_dot = (time.time() - t0) / nent
+    tot = 0.0
+    for g in runs:
+        S2 = g.plan.Smax ** 2
+        rhs = 2 * NQ_PER_SIGN * len(g.subs[0][4])
+        tot += rhs * S2 * (g.plan.cauchy_terms() * t_term + 2 * sum(l.T for l in g.plan.levels if l) * t_dot)
+    blocks = sum(g.blocks() for g in runs)
+    return {"value": blocks / tot, "unit": "blocks/s", "cores": 1, "kind": "port",
+            "sample": f"oracle mpf_ref Cauchy terms ({t_term * 1e6:.1f} us each) and pole-step dot products "
+                      f"({t_dot * 1e6:.1f} us each) at 34 limbs for {seconds:.0f} s on one core, scaled by the "
+                      f"workload's recursion term counts (derivative and output stages not counted)"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+    import torch
+    from blocksvibe_amd import mpf
+    L = mpf.limbs_for_precision(PREC)
+    groups = group_list(args.groups)
+    t0 = time.time()
+    runs = [GroupRun(a, b, L, seed=i + 1) for i, (a, b) in enumerate(groups)]
+    torch.cuda.synchronize()
+    setup_s = time.time() - t0
+    print(f"# setup of {len(runs)} operator groups: {setup_s:.1f} s", file=sys.stderr, flush=True)
+    for w in range(args.warmup):
+        for g in runs:
+            g.run()
+        torch.cuda.synchronize()
+        print(f"# warmup step {w} done", file=sys.stderr, flush=True)
+    prof = {}
+    torch.cuda.synchronize()
+    t0 = time.time()
+    for s in range(args.steps):
+        for gi, g in enumerate(runs):
+            g.run()
+            torch.cuda.synchronize()
+            print(f"# step {s} group {g.tj12},{g.tj43} done at {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    dt = time.time() - t0
+    blocks = sum(g.blocks() for g in runs) * args.steps
+    res = {"metric": "blocks/s at nmax=6, 1024-bit (batched GPU chain)", "value": blocks / dt, "unit": "blocks/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32 limbs (1086-bit mpf)",
+           "data": "synthetic (seeded random M matrices and h_inf of the real shapes)",
+           "config": {"workload": "nmax=6 spin-2 externals: 25 operator groups x 22 structures x 6 lambda orders, "
+                                  "order 60, kept 14, 1024 bits, xt",
+                      "groups": len(runs), "tasks": 22 * len(runs), "blocks_per_step": blocks // args.steps}}
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline(runs)
+    print(json.dumps(res), flush=True)
+    if args.json_out:
+        with open(args.json_out, "w") as f:
+            json.dump(res, f)
+
+
+if __name__ == "__main__":
+    main()

@@ -300,6 +300,47 @@ __device__ __forceinline__ void div_small(const u32 (&x)[L], u32 d, u32 (&q)[L +
 }
 
 
+// acc += a * x, a an L-limb two's complement value, x an L-limb magnitude, acc NW >= 2L + 1 limbs
+// (wrapping).  Product scanning: the column sums of the unsigned product bits(a) * x are accumulated in
+// 96 bits and folded into acc one limb at a time (no product array, no carry propagation per row); a
+// negative a contributes bits(a) = a + 2^(32L), corrected by subtracting x at limb offset L.
+template <int L, int NW>
+__device__ __forceinline__ void mac_signed(u32 (&acc)[NW], const u32 (&a)[L], const u32 (&x)[L]) {
+  u64 t01 = 0;
+  u32 t2 = 0;
+#pragma unroll
+  for (int s = 0; s < 2 * L - 1; ++s) {
+#pragma unroll
+    for (int i = (s - L + 1 > 0 ? s - L + 1 : 0); i <= (s < L - 1 ? s : L - 1); ++i) {
+      const u64 prod = (u64)a[i] * x[s - i];
+      const u64 n = t01 + prod;
+      t2 += (n < prod) ? 1u : 0u;
+      t01 = n;
+    }
+    const u64 n = t01 + (u64)acc[s];
+    t2 += (n < t01) ? 1u : 0u;
+    acc[s] = (u32)n;
+    t01 = (n >> 32) | ((u64)t2 << 32);
+    t2 = 0;
+  }
+  u64 c = t01;
+#pragma unroll
+  for (int s = 2 * L - 1; s < NW; ++s) {
+    const u64 n = (u64)acc[s] + (c & 0xFFFFFFFFull);
+    acc[s] = (u32)n;
+    c = (c >> 32) + (n >> 32);
+  }
+  if (a[L - 1] >> 31) {                      // acc -= x * 2^(32 L)
+    u32 b = 0;
+#pragma unroll
+    for (int s = L; s < NW; ++s) {
+      const u64 v = (u64)acc[s] - (u64)(s - L < L ? x[s - L] : 0u) - (u64)b;
+      acc[s] = (u32)v;
+      b = (u32)(v >> 63);
+    }
+  }
+}
+
 // out = norm(sum_j A[j] * X[j]) over K = S terms: A pre-aligned (mantissas a_j, exponent ea), X entries
 // aligned here (over j).  Shared by both pole-step products.
 template <int L>
@@ -321,9 +362,6 @@ __device__ __forceinline__ i64 small_dot(const int32_t* __restrict__ am, i64 aps
   u32 acc[2 * L + 2];
   mp::zero(acc);
   for (int j = 0; j < S; ++j) {
-    u32 a[L];
-    mp::load(a, am, aps, abase + j);
-    if (mp::is_zero(a)) continue;
     const i64 idx = xbase + j * xstride;
     u32 x[L];
     mp::load(x, xm, xps, idx);
@@ -332,11 +370,10 @@ __device__ __forceinline__ i64 small_dot(const int32_t* __restrict__ am, i64 aps
     const i64 sh = E - xe[idx];
     if (sh >= 32 * L) continue;
     mp::shr(x, (int)sh);
-    const bool na = mp::to_mag(a);
-    u32 p[2 * L];
-    mp::mul_full(a, x, p);
-    if (na != nx) mp::sub_from(acc, p);
-    else mp::add_into(acc, p);
+    u32 a[L];
+    mp::load(a, am, aps, abase + j);
+    if (nx) mp::neg(a);                     // sign of x folded into a (|a| < 2^(32L-2): no overflow)
+    mac_signed(acc, a, x);
   }
   return mp::norm<L, 2 * L + 2>(acc, ea + E, out);
 }

@@ -260,3 +260,23 @@ def random(shape, L, spread=2, generator=None, device=None):
     device = device or dev()
     return MPF(torch.from_numpy(m.view(np.int32)).reshape((L,) + tuple(shape)).to(device),
                torch.from_numpy(e).reshape(tuple(shape)).to(device))
+
+
+def random_device(shape, L, spread=2, seed=0, device=None):
+    """random normalised MPF generated on the device (uniform mantissa bits with bitlen exactly NB, random
+    sign, exponent spread +-spread around 2^0): synthetic inputs without a host round trip"""
+    device = device or dev()
+    n = int(np.prod(shape)) if shape else 1
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    m = torch.randint(-(1 << 31), (1 << 31) - 1, (L, n), dtype=torch.int32, device=device, generator=g)
+    m[L - 1] = (m[L - 1] & 0x3FFFFFFF) | 0x20000000                       # magnitude bitlen exactly NB
+    neg = torch.randint(0, 2, (n,), device=device, generator=g).bool()
+    W = torch.cat([m, torch.zeros((1, n), dtype=torch.int32, device=device)], 0)
+    e = torch.randint(-spread, spread + 1, (n,), dtype=torch.int64, device=device, generator=g) - (32 * L - 2)
+    out = norm_wide(W, e, L)                                              # already normalised: exact copy
+    sgn = torch.zeros((L, n), dtype=torch.int32, device=device)           # +-2^(NB-1) * 2^-(NB-1)
+    sgn[L - 1] = torch.where(neg, torch.tensor(-(1 << 29), dtype=torch.int32, device=device),
+                             torch.tensor(1 << 29, dtype=torch.int32, device=device))
+    one = MPF(sgn, torch.full((n,), -(32 * L - 3), dtype=torch.int64, device=device))
+    return mul(out, one).reshape(*shape)

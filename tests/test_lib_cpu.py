@@ -1,0 +1,31 @@
+"""The C-ABI library (no GPU needed): libb3dhip.so loads, exports every entry point declared in
+include/b3dhip.h with the header's ABI version, and reports its compiled limb counts."""
+import ctypes
+
+import pytest
+
+
+def test_exports_every_declared_symbol():
+    from blocksvibe_amd import _lib
+    lib = ctypes.CDLL(_lib.LIBPATH)
+    decl = _lib.declarations()
+    assert len(decl) >= 10
+    for name, _, _ in decl:
+        assert hasattr(lib, name), name
+
+
+def test_abi_and_limbs():
+    import re
+    from blocksvibe_amd import _lib
+    v = int(re.search(r"#define B3D_ABI_VERSION (\d+)", open(_lib.HEADER).read()).group(1))
+    assert _lib.lib().b3d_abi_version() == v
+    for L in (10, 18, 26, 34, 42, 50):
+        assert _lib.lib().b3d_supported_limbs(L) == 1
+    assert _lib.lib().b3d_supported_limbs(7) == 0
+
+
+def test_limb_rule():
+    from blocksvibe_amd.mpf import limbs_for_precision
+    # reference cli.py:270-274: 256 -> 10, 512 -> 18, 768 -> 26, 1024 -> 34, 1280 -> 42, 1536 -> 50
+    assert [limbs_for_precision(p) for p in (256, 512, 768, 1024, 1280, 1536)] == [10, 18, 26, 34, 42, 50]
+    assert limbs_for_precision(384) == 14 and limbs_for_precision(200) == 10
