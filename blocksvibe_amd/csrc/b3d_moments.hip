@@ -51,52 +51,14 @@ __global__ __launch_bounds__(256) void k_mom_stage(const int32_t* __restrict__ r
 template <int L, int KLM>
 __global__ __launch_bounds__(256) void k_mom_flush(const int32_t* __restrict__ sm, const i64* __restrict__ se, i64 sps,
                                                    int W, int m0, const int32_t* __restrict__ srow,
-                                                   const uint32_t* __restrict__ NK, i64 nkps,
+                                                   const uint32_t* __restrict__ NK, i64 nkps, int KL,
                                                    const int32_t* __restrict__ NKb, int32_t* __restrict__ vm,
                                                    i64* __restrict__ ve, i64 vps, i64 nrows, i64 N, int R1) {
-  const int NBITS = 32 * L - 2;
   for (i64 t = blockIdx.x * (i64)blockDim.x + threadIdx.x; t < nrows * N * R1; t += (i64)gridDim.x * blockDim.x) {
     const int k = (int)(t % R1);
     const i64 c = (t / R1) % N, row = t / ((i64)R1 * N);
-    const int s = srow[row];
-    i64 E = B3D_EZ;
-    for (int w = 0; w < W; ++w) {
-      const i64 o = ((i64)w * nrows + row) * N + c;
-      if (sm[(L - 1) * sps + o] == 0) continue;
-      const i64 x = se[o] + NBITS + NKb[(i64)(m0 + w + s) * R1 + k];
-      E = x > E ? x : E;
-    }
-    if (E == B3D_EZ) continue;
-    const i64 T = E - 32 * (L + 1);
-    u32 acc[L + 3];
-    mp::zero(acc);
-    for (int w = 0; w < W; ++w) {
-      const i64 o = ((i64)w * nrows + row) * N + c;
-      u32 y[L];
-      mp::load(y, sm, sps, o);
-      if (y[L - 1] == 0u) continue;
-      const bool ny = mp::to_mag(y);
-      const i64 ni = (i64)(m0 + w + s) * R1 + k;
-      u32 nk[KLM];
-#pragma unroll
-      for (int q = 0; q < KLM; ++q) nk[q] = NK[q * nkps + ni];
-      u32 p[L + KLM];
-      mp::mul_full(y, nk, p);
-      const i64 sh = T - se[o];
-      if (sh >= 0) {
-        if (sh >= 32 * (L + KLM)) continue;
-        mp::shr(p, (int)sh);
-      } else {
-        mp::shl(p, (int)(-sh));
-      }
-      u32 q[L + 2];
-#pragma unroll
-      for (int z = 0; z < L + 2; ++z) q[z] = z < L + KLM ? p[z] : 0u;
-      if (ny) mp::sub_from(acc, q);
-      else mp::add_into(acc, q);
-    }
     u32 r[L];
-    const i64 er = mp::norm<L, L + 3>(acc, T, r);
+    const i64 er = mp::mom_window<L, KLM>(sm, se, sps, W, nrows, N, row, c, m0 + srow[row], k, R1, NK, nkps, KL, NKb, r);
     if (r[L - 1] == 0u) continue;
     const i64 vo = (row * N + c) * R1 + k;
     u32 v[L], out[L];
@@ -127,11 +89,11 @@ int b3d_moments_flush(int L, int KL, const int32_t* sm, const int64_t* se, int64
   hipStream_t s = (hipStream_t)stream;
   const int64_t n = nrows * N * R1;
   if (KL <= 4) {
-    B3D_DISPATCH_L(L, k_mom_flush<LL, 4><<<grid_for(n, 256), 256, 0, s>>>(sm, se, sps, W, m0, srow, NK, nkps, NKb,
-                                                                           vm, ve, vps, nrows, N, R1));
+    B3D_DISPATCH_L(L, k_mom_flush<LL, 4><<<grid_for(n, 256), 256, 0, s>>>(sm, se, sps, W, m0, srow, NK, nkps, KL,
+                                                                           NKb, vm, ve, vps, nrows, N, R1));
   } else if (KL <= 8) {
-    B3D_DISPATCH_L(L, k_mom_flush<LL, 8><<<grid_for(n, 256), 256, 0, s>>>(sm, se, sps, W, m0, srow, NK, nkps, NKb,
-                                                                           vm, ve, vps, nrows, N, R1));
+    B3D_DISPATCH_L(L, k_mom_flush<LL, 8><<<grid_for(n, 256), 256, 0, s>>>(sm, se, sps, W, m0, srow, NK, nkps, KL,
+                                                                           NKb, vm, ve, vps, nrows, N, R1));
   } else {
     b3d::set_error("b3d_moments_flush: n^k needs more than 8 limbs");
     return 2;

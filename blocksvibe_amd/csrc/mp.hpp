@@ -428,4 +428,56 @@ __device__ __forceinline__ i64 cauchy_entry(const int32_t* __restrict__ rm, cons
   return add<L>(c, ec, hv, he[hidx], r);
 }
 
+// One windowed moment: sum_w y_w n_w^k for the staged entries y_w = stage[w, row, c] (w < W), n_w = n0 + w,
+// exact n^k from NK (KL limb planes, entry n * R1 + k, bit lengths NKb).  E = max_w (e_w + NB + bitlen n_w^k)
+// bounds the top bit; every product is put on the grid 2^T, T = E - 32 (L + 1) (truncated), summed exactly
+// in L + 3 limbs and rounded once.  Returns the exponent (EZ and r = 0 if every y_w is zero).
+template <int L, int KLM>
+__device__ __forceinline__ i64 mom_window(const int32_t* __restrict__ sm, const i64* __restrict__ se, i64 sps, int W,
+                                          i64 nrows, i64 N, i64 row, i64 c, int n0, int k, int R1,
+                                          const uint32_t* __restrict__ NK, i64 nkps, int KL,
+                                          const int32_t* __restrict__ NKb, u32 (&r)[L]) {
+  const int NBITS = 32 * L - 2;
+  i64 E = B3D_EZ;
+  for (int w = 0; w < W; ++w) {
+    const i64 o = ((i64)w * nrows + row) * N + c;
+    if (sm[(L - 1) * sps + o] == 0) continue;
+    const i64 x = se[o] + NBITS + NKb[(i64)(n0 + w) * R1 + k];
+    E = x > E ? x : E;
+  }
+  if (E == B3D_EZ) {
+    zero(r);
+    return B3D_EZ;
+  }
+  const i64 T = E - 32 * (L + 1);
+  u32 acc[L + 3];
+  zero(acc);
+  for (int w = 0; w < W; ++w) {
+    const i64 o = ((i64)w * nrows + row) * N + c;
+    u32 y[L];
+    load(y, sm, sps, o);
+    if (y[L - 1] == 0u) continue;
+    const bool ny = to_mag(y);
+    const i64 ni = (i64)(n0 + w) * R1 + k;
+    u32 nk[KLM];
+#pragma unroll
+    for (int q = 0; q < KLM; ++q) nk[q] = q < KL ? NK[q * nkps + ni] : 0u;  // the table has KL limb planes
+    u32 p[L + KLM];
+    mul_full(y, nk, p);
+    const i64 sh = T - se[o];
+    if (sh >= 0) {
+      if (sh >= 32 * (L + KLM)) continue;
+      shr(p, (int)sh);
+    } else {
+      shl(p, (int)(-sh));
+    }
+    u32 q[L + 2];
+#pragma unroll
+    for (int z = 0; z < L + 2; ++z) q[z] = z < L + KLM ? p[z] : 0u;
+    if (ny) sub_from(acc, q);
+    else add_into(acc, q);
+  }
+  return norm<L, L + 3>(acc, T, r);
+}
+
 }  // namespace mp

@@ -4,7 +4,6 @@ initial data with several right-hand sides (reference tests/test_blocks_recursio
   * within 2^-(32L - 64) of the exact rational recursion relative to each (pole, spin) block;
   * with slot recycling (reuse=True) every residue, captured right after its level, is identical.
 """
-import math
 import random
 from fractions import Fraction as F
 
@@ -98,7 +97,7 @@ def test_recursion_bits_and_exact(case):
             ex = exact.get((p, m, j, q, a, b))
             if ex is not None and blk[(p, m, j, q)]:
                 worst = max(worst, abs(R.value(*g) - ex) / blk[(p, m, j, q)])
-    bits_err = -math.log2(worst) if worst else float("inf")
+    bits_err = (worst.denominator.bit_length() - worst.numerator.bit_length()) if worst else float("inf")
     print(f"max error relative to the (pole, spin) block: 2^-{bits_err:.1f} (bound 2^-{32 * L - 64})")
     assert bits_err > 32 * L - 64
     # reuse: every residue captured after its level equals the non-reuse result
