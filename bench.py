@@ -36,6 +36,7 @@ def parse():
     ap.add_argument("--groups", default="all", help="all | heavy (2j12 = 2j43 = 8 only) | comma list a:b")
     ap.add_argument("--plain", action="store_true", help="also time the plain per-task path on one group")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
+    ap.add_argument("--serve", type=float, default=30.0, help="seconds of the mixed serving stream (0: skip)")
     ap.add_argument("--json-out", default=None)
     return ap.parse_args()
 
@@ -48,6 +49,16 @@ def group_list(spec):
     return [tuple(int(x) for x in g.split(":")) for g in spec.split(",")]
 
 
+_OUTPUTS = {}
+
+
+def output_map(p0, L):
+    from blocksvibe_amd.output import Output
+    if (p0, L) not in _OUTPUTS:
+        _OUTPUTS[(p0, L)] = Output("xt", LAMBDA, p0, D12, L)
+    return _OUTPUTS[(p0, L)]
+
+
 class GroupRun:
     """one operator group: constants and resident inputs for both signs"""
 
@@ -55,7 +66,6 @@ class GroupRun:
         import torch
         from blocksvibe_amd import mpf
         from blocksvibe_amd.derivatives import Derivs
-        from blocksvibe_amd.output import Output
         from blocksvibe_amd.plan import Plan
         from blocksvibe_amd.recursion import Recursion
         self.tj12, self.tj43, self.L = tj12, tj43, L
@@ -63,10 +73,10 @@ class GroupRun:
         self.rec = Recursion(self.plan, None, L, seed=seed)
         self.subs = []
         Sm, Jh = self.plan.Smax, self.plan.Jh
+        der = Derivs(self.plan, LAMBDA, L, kept_pole_order=KEPT)     # R = Lambda covers every lambda order
         for sign in (1, -1):
             p0 = (1 - sign) // 2
-            der = Derivs(self.plan, LAMBDA - p0, L, kept_pole_order=KEPT)
-            out = Output("xt", LAMBDA, p0, D12, L)
+            out = output_map(p0, L)
             ns = list(range(p0, LAMBDA + 1, 2))
             hs = [mpf.random_device((2, ORDER + 1, Jh, Sm * Sm, NQ_PER_SIGN), L, seed=seed * 100 + 10 * p0 + i)
                   for i, n in enumerate(ns)]
@@ -108,6 +118,80 @@ class GroupRun:
                 out.apply_per_order(nums)
                 count += 1
         return count
+
+
+def serving_stream(seconds, seed=7):
+    """Mixed request stream (the serving configuration): single-task requests drawn from external spins
+    {0, 1/2, 1, 2}, nmax 2..8 (Lambda = 2 nmax - 1, order 20..60, kept 2 nmax + 2), precision
+    {256, 512, 768, 1024, 1280} bits, internal spins up to 20, several Delta_12/Delta_43 operators per shape
+    (distinct M matrices), both signs; Poisson arrivals at ~0.8 of the capacity measured by a first burst.
+    Operators and maps are built when first seen (a warm-up pass over the pool precedes the stream)."""
+    import random
+    import numpy as np
+    import torch
+    from blocksvibe_amd import mpf
+    from blocksvibe_amd.plan import Plan
+    from blocksvibe_amd.serve import Request, Server
+    rng = random.Random(seed)
+    order_of = {2: 20, 3: 30, 4: 40, 5: 50, 6: 60, 7: 60, 8: 60}
+    pool = []
+    for i in range(10):
+        ext = rng.choice([0, 1, 2, 4])
+        tj12 = rng.choice(range(ext % 2, 2 * ext + 1, 2))
+        tj43 = rng.choice(range(ext % 2, 2 * ext + 1, 2))
+        nmax = rng.randint(2, 8)
+        prec = rng.choice([256, 512, 768, 1024, 1280])
+        L = mpf.limbs_for_precision(prec)
+        tjmax = rng.choice([8, 12, 16, 20]) + tj12 % 2
+        Lam, order = 2 * nmax - 1, order_of[nmax]
+        pp = ((0, 0), (1, 1))
+        plan = Plan(tj12, tj43, tjmax, order, pp)
+        for variant in range(3):                                   # distinct Delta_12, Delta_43
+            pool.append(((tj12, tj43, tjmax, order, Lam, 2 * nmax + 2, L, pp, 10 * i + variant), plan))
+    hcache = {}
+
+    def make(entry, sign, k):
+        key, plan = entry
+        p0 = (1 - sign) // 2
+        ns = list(range(p0, key[4] + 1, 2))
+        hk = (key[:8], sign, k % 2)
+        if hk not in hcache:
+            hcache[hk] = [mpf.random_device((2, key[3] + 1, plan.Jh, plan.Smax ** 2, 1), key[6], seed=k * 31 + n)
+                          for n in ns]
+        return Request(key, sign, hcache[hk], coords=("xt",), d12="0")
+
+    srv = Server(streams=2)
+    t0 = time.time()
+    warm = [srv.submit(make(e, s, 0)) for e in pool for s in (1, -1)]
+    for r in warm:
+        r.wait(600)
+    warm_s = time.time() - t0
+    burst = [srv.submit(make(rng.choice(pool), rng.choice((1, -1)), k)) for k in range(40)]
+    t0 = time.time()
+    for r in burst:
+        r.wait(600)
+    cap = len(burst) / (time.time() - t0)
+    srv.completed.clear()
+    rate = 0.8 * cap
+    reqs, t0, nxt, k = [], time.time(), 0.0, 0
+    while time.time() - t0 < seconds:
+        now = time.time() - t0
+        if now >= nxt:
+            reqs.append(srv.submit(make(rng.choice(pool), rng.choice((1, -1)), k)))
+            k += 1
+            nxt += rng.expovariate(rate)
+        else:
+            time.sleep(min(0.005, nxt - now))
+    for r in reqs:
+        r.wait(600)
+    wall = time.time() - t0
+    srv.close()
+    lat = sorted(r.t_done - r.t_arrival for r in reqs)
+    q = lambda f: lat[min(len(lat) - 1, int(f * len(lat)))]
+    return {"requests": len(reqs), "rate_per_s": rate, "capacity_per_s": cap, "p50_ms": q(0.5) * 1e3,
+            "p99_ms": q(0.99) * 1e3, "blocks_per_s": sum(r.blocks for r in reqs) / wall,
+            "mean_batch": float(np.mean(srv.batch_sizes)) if srv.batch_sizes else 0.0, "warmup_s": warm_s,
+            "operators": len(pool)}
 
 
 def cpu_baseline(runs, seconds=12.0):
@@ -185,6 +269,9 @@ def main():
            "config": {"workload": "nmax=6 spin-2 externals: 25 operator groups x 22 structures x 6 lambda orders, "
                                   "order 60, kept 14, 1024 bits, xt",
                       "groups": len(runs), "tasks": 22 * len(runs), "blocks_per_step": blocks // args.steps}}
+    if args.serve > 0:
+        res["serving"] = serving_stream(args.serve)
+        print(f"# serving: {res['serving']}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(runs)
     print(json.dumps(res), flush=True)

@@ -77,11 +77,19 @@ def numerator_columns(tj, tj12, tj43, order, kept, R):
     keep = new[nprot:]
     size = len(keep)
     h = size // 2
+    # S_ki = (y'_k / y)^h prod_{m != k} (y - y'_m) / (y'_k - y'_m): integer weights over the common
+    # denominator y^h D, D = lcm |prod_{m != k} (y'_k - y'_m)|
     Dk = [1] * size
     for k in range(size):
         for m in range(size):
             if m != k:
                 Dk[k] *= keep[k] - keep[m]
+    D = 1
+    for d in Dk:
+        D = lcm(D, abs(d))
+    fac = [keep[k] ** h * (D // Dk[k]) for k in range(size)]           # exact: Dk divides D up to sign
+    Q = np.array([comp[nprot + k] + [0] * (npj - len(comp[nprot + k])) for k in range(size)], dtype=object) \
+        if size else None
     for i, y in enumerate(roots):
         if y == 0:
             cols[i] = [Fraction(c) * Fraction(2) ** (k - (npj - 1)) for k, c in enumerate(comp[new.index(0)])]
@@ -89,17 +97,16 @@ def numerator_columns(tj, tj12, tj43, order, kept, R):
         if size == 0:
             cols[i] = [Fraction(0)]
             continue
-        poly = [Fraction(0)] * npj
+        diff = [y - km for km in keep]
+        pre, suf = [1] * (size + 1), [1] * (size + 1)
         for k in range(size):
-            # S_ki = (y'_k / y)^h prod_{m != k} (y - y'_m) / (y'_k - y'_m)
-            num = Fraction(keep[k]) ** h
-            for m in range(size):
-                if m != k:
-                    num *= y - keep[m]
-            ski = num / (Fraction(y) ** h * Dk[k])
-            for d, c in enumerate(comp[nprot + k]):
-                poly[d] += ski * c
-        cols[i] = [c * Fraction(2) ** (d - (npj - 1)) for d, c in enumerate(poly)]
+            pre[k + 1] = pre[k] * diff[k]
+        for k in range(size - 1, -1, -1):
+            suf[k] = suf[k + 1] * diff[k]
+        w = np.array([fac[k] * pre[k] * suf[k + 1] for k in range(size)], dtype=object)
+        poly = w.dot(Q)
+        den = y ** h * D
+        cols[i] = [Fraction(int(c), den) * Fraction(2) ** (d - (npj - 1)) for d, c in enumerate(poly)]
     return cols, npj
 
 

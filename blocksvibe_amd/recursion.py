@@ -64,6 +64,26 @@ class Recursion:
         self._constants()
         self._upload_levels()
 
+    def with_matrices(self, M, seed=0):
+        """a Recursion of the same plan and limb count for other M matrices (another Delta_12, Delta_43 or
+        external spins), sharing the uploaded level tables and Q values: only the M-dependent tables
+        (R rows aligned, q L products aligned) are rebuilt"""
+        r = object.__new__(Recursion)
+        r.__dict__.update(self.__dict__)
+        Sm, L, nM = self.plan.Smax, self.L, len(self.plan.Mkeys)
+        if M is None:
+            r.Mtab = mpf.random((max(nM, 1), Sm, Sm), L, spread=1, generator=np.random.default_rng(seed))
+        else:
+            vals = [(0, mpf.EZ)] * (max(nM, 1) * Sm * Sm)
+            for key, i in self.plan.Mkeys.items():
+                mat = M.get(key) if isinstance(M, dict) else M.pairs(key)
+                for a, row in enumerate(mat or []):
+                    for b, x in enumerate(row):
+                        vals[(i * Sm + a) * Sm + b] = number_pair(x, L)
+            r.Mtab = mpf.from_pairs(vals, L, (max(nM, 1), Sm, Sm))
+        r._constants()
+        return r
+
     def _constants(self):
         plan, Sm, L = self.plan, self.plan.Smax, self.L
         self.Ralign_m, self.Ralign_e = mpf.align_rows(self.Mtab)            # (L, nM, S, S), (nM, S)

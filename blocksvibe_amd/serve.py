@@ -1,0 +1,222 @@
+"""Device-resident serving loop for blocks_3d requests (continuous batching).
+
+Requests (one blocks_3d task each: operator, four-point sign, lambda orders, h_inf) arrive at any time.
+Worker threads, one per HIP stream, repeatedly take the oldest waiting request and every waiting request
+compatible with it (same operator, sign and lambda orders), up to what the HBM budget still admits, and
+run them as one batch: one recursion per lambda order over all their structures, the moments, the
+numerator GEMMs and the write_output GEMM.  Requests that arrive while batches are on the GPU join the
+next batch as soon as a worker and memory are free, and batches on different streams overlap on the
+device.  Plans, uploaded level tables, operator constants (M, qL, R tables) and write_output maps are cached
+on the GPU across requests.
+
+Every kernel of the chain treats each right-hand-side column independently (per-column alignment, one
+rounding per entry, A operands that do not depend on the batch), so a request's result is bit-identical
+whether it runs alone or in any mix (tests/test_gpu_serve.py).
+
+Analogue of the reference's spool batching of concurrent invocations and per-GPU lock (reference
+blocksvibe/cli.py:238-265, 312-321, 569-637), in one process.
+"""
+import collections
+import threading
+import time
+
+import torch
+
+from . import mpf
+from .derivatives import Derivs
+from .output import Output
+from .plan import Plan
+from .recursion import Recursion
+
+
+class Request:
+    """one task: operator (tj12, tj43, tjmax, order, Lambda, kept, L, pp, op_id), sign, h_inf per lambda order
+    (device MPF (2, order + 1, Jh, S^2, 1)), output coordinates and Delta_1+2.  op_id distinguishes operators
+    with the same shape but different M matrices (Delta_12, Delta_43, external spins)."""
+    __slots__ = ("key", "sign", "hs", "coords", "d12", "spins", "t_arrival", "t_start", "t_done", "result", "done",
+                 "error", "blocks")
+
+    def __init__(self, key, sign, hs, coords=("xt",), d12="0", spins=None):
+        self.key, self.sign, self.hs, self.coords, self.d12, self.spins = key, sign, hs, tuple(coords), d12, spins
+        self.t_arrival = self.t_start = self.t_done = None
+        self.result, self.error = None, None
+        self.done = threading.Event()
+        self.blocks = len(spins) if spins else len(range(key[0] % 2, key[2] + 1, 2))
+
+    def wait(self, timeout=None):
+        self.done.wait(timeout)
+        if self.error:
+            raise self.error
+        return self.result
+
+
+class _Operator:
+    """cached device state of one operator: the shape-dependent part (plan, level tables, Q values,
+    derivative constants) is shared by every operator of the same shape; only the M-dependent tables are
+    built per operator"""
+
+    def __init__(self, key, shapes, seed):
+        tj12, tj43, tjmax, order, Lam, kept, L, pp, op_id = key
+        sk = (tj12, tj43, tjmax, order, pp, L)
+        if sk not in shapes:
+            plan = Plan(tj12, tj43, tjmax, order, pp, reuse=True)
+            shapes[sk] = {"plan": plan, "rec": Recursion(plan, None, L, seed=0), "derivs": {}}
+        self.shape = shapes[sk]
+        self.plan = self.shape["plan"]
+        self.rec = self.shape["rec"].with_matrices(None, seed=seed)
+        self.key = key
+
+    def derivs_for(self, p0):
+        tj12, tj43, tjmax, order, Lam, kept, L, pp, op_id = self.key
+        dk = (Lam - p0, kept)
+        if dk not in self.shape["derivs"]:
+            self.shape["derivs"][dk] = Derivs(self.plan, Lam - p0, L, kept_pole_order=kept)
+        return self.shape["derivs"][dk]
+
+    def per_rhs_bytes(self, p0):
+        d = self.derivs_for(p0)
+        pl, L = self.plan, self.key[6]
+        S2, nb = pl.Smax ** 2, 4 * L + 8
+        return S2 * nb * ((pl.nphys + 1) + (d.R + 1 + 8) * 2 * d.Jo * d.NP)
+
+    def h_bytes(self):
+        pl, L = self.plan, self.key[6]
+        return pl.Smax ** 2 * (4 * L + 8) * 2 * (pl.order + 1) * pl.Jh
+
+
+class Server:
+    def __init__(self, hbm_budget=None, streams=2, max_batch=64):
+        if hbm_budget is None:
+            free, _ = torch.cuda.mem_get_info()
+            hbm_budget = int(0.5 * free)
+        self.budget = hbm_budget
+        self.in_use = 0
+        self.max_batch = max_batch
+        self.lock = threading.Condition()
+        self.queue = collections.deque()
+        self.ops, self.plans, self.outs = {}, {}, {}
+        self.setup_lock = threading.Lock()
+        self.stop = False
+        self.completed = []
+        self.batch_sizes = []
+        self.streams = [torch.cuda.Stream() for _ in range(streams)]
+        self.workers = [threading.Thread(target=self._work, args=(s,), daemon=True) for s in self.streams]
+        for w in self.workers:
+            w.start()
+
+    # ------------------------------------------------------------------------------------------ API
+    def submit(self, req):
+        req.t_arrival = time.time()
+        with self.lock:
+            self.queue.append(req)
+            self.lock.notify_all()
+        return req
+
+    def close(self):
+        with self.lock:
+            self.stop = True
+            self.lock.notify_all()
+        for w in self.workers:
+            w.join()
+
+    def latencies(self):
+        return sorted(r.t_done - r.t_arrival for r in self.completed)
+
+    # ------------------------------------------------------------------------------------------ internals
+    def _operator(self, key):
+        with self.setup_lock:
+            if key not in self.ops:
+                self.ops[key] = _Operator(key, self.plans, seed=len(self.ops) + 1)
+            return self.ops[key]
+
+    def _output(self, coord, Lam, p0, d12, L):
+        k = (coord, Lam, p0, d12, L)
+        with self.setup_lock:
+            if k not in self.outs:
+                self.outs[k] = Output(coord, Lam, p0, d12, L)
+            return self.outs[k]
+
+    def _take_batch(self):
+        """oldest request plus compatible ones within the HBM budget (caller holds the lock)"""
+        while True:
+            if self.stop and not self.queue:
+                return None, 0
+            if self.queue:
+                head = self.queue[0]
+                op = self.ops.get(head.key)
+                if op is None:                              # first request of an operator: build it unlocked
+                    self.lock.release()
+                    try:
+                        self._operator(head.key)
+                    finally:
+                        self.lock.acquire()
+                    continue
+                p0 = (1 - head.sign) // 2
+                with self.setup_lock:
+                    per = op.per_rhs_bytes(p0) + op.h_bytes() * len(head.hs)
+                room = self.budget - self.in_use
+                if per <= room or self.in_use == 0:
+                    batch = [head]
+                    nmax = max(1, min(self.max_batch, room // per if per else self.max_batch))
+                    for r in list(self.queue)[1:]:
+                        if len(batch) >= nmax:
+                            break
+                        if r.key == head.key and r.sign == head.sign and len(r.hs) == len(head.hs):
+                            batch.append(r)
+                    for r in batch:
+                        self.queue.remove(r)
+                    cost = per * len(batch)
+                    self.in_use += cost
+                    self.batch_sizes.append(len(batch))
+                    return batch, cost
+            self.lock.wait(0.05)
+
+    def _work(self, stream):
+        while True:
+            with self.lock:
+                batch, cost = self._take_batch()
+            if batch is None:
+                return
+            try:
+                with torch.cuda.stream(stream):
+                    self._run(batch)
+                    stream.synchronize()
+                t = time.time()
+                for r in batch:
+                    r.t_done = t
+            except Exception as e:          # surface to the callers
+                for r in batch:
+                    r.error = e
+                    r.t_done = time.time()
+            with self.lock:
+                self.in_use -= cost
+                self.completed.extend(batch)
+                self.lock.notify_all()
+            for r in batch:
+                r.done.set()
+
+    def _run(self, batch):
+        t0 = time.time()
+        head = batch[0]
+        op = self._operator(head.key)
+        tj12, tj43, tjmax, order, Lam, kept, L, pp, op_id = head.key
+        p0 = (1 - head.sign) // 2
+        der = op.derivs_for(p0)
+        for r in batch:
+            r.t_start = t0
+        nums = []
+        for i in range(len(head.hs)):
+            n = p0 + 2 * i
+            h = mpf.cat([r.hs[i] for r in batch], 4) if len(batch) > 1 else head.hs[i]
+            dn = der.for_order(Lam - n)
+            dn.start(len(batch))
+            op.rec.run(h, hook=dn.accumulate)
+            nums.append(dn.finish(h))
+        coords = sorted({(c, r.d12) for r in batch for c in r.coords})
+        outs = {}
+        for c, d in coords:
+            o = self._output(c, Lam, p0, d, L)
+            outs[(c, d)] = o.apply_per_order(nums)              # (Jo, 2, S2, nq, outputs, Dmax)
+        for q, r in enumerate(batch):
+            r.result = {k: mpf.MPF(v.m[:, :, :, :, q].contiguous(), v.e[:, :, :, q].contiguous())
+                        for k, v in outs.items() if (k[0] in r.coords and k[1] == r.d12)}
