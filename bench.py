@@ -163,8 +163,9 @@ def serving_stream(seconds, seed=7):
     srv = Server(streams=2)
     t0 = time.time()
     warm = [srv.submit(make(e, s, 0)) for e in pool for s in (1, -1)]
-    for r in warm:
+    for i, r in enumerate(warm):
         r.wait(600)
+        print(f"# serving warm-up {i + 1}/{len(warm)} at {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
     warm_s = time.time() - t0
     burst = [srv.submit(make(rng.choice(pool), rng.choice((1, -1)), k)) for k in range(40)]
     t0 = time.time()
@@ -172,6 +173,7 @@ def serving_stream(seconds, seed=7):
         r.wait(600)
     cap = len(burst) / (time.time() - t0)
     srv.completed.clear()
+    print(f"# serving capacity {cap:.2f} requests/s; streaming for {seconds:.0f} s", file=sys.stderr, flush=True)
     rate = 0.8 * cap
     reqs, t0, nxt, k = [], time.time(), 0.0, 0
     while time.time() - t0 < seconds:
@@ -242,7 +244,10 @@ def main():
     L = mpf.limbs_for_precision(PREC)
     groups = group_list(args.groups)
     t0 = time.time()
-    runs = [GroupRun(a, b, L, seed=i + 1) for i, (a, b) in enumerate(groups)]
+    runs = []
+    for i, (a, b) in enumerate(groups):
+        runs.append(GroupRun(a, b, L, seed=i + 1))
+        print(f"# set up group {a},{b} ({i + 1}/{len(groups)}) at {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     setup_s = time.time() - t0
     print(f"# setup of {len(runs)} operator groups: {setup_s:.1f} s", file=sys.stderr, flush=True)

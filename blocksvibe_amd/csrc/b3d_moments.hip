@@ -36,11 +36,11 @@ __global__ __launch_bounds__(256) void k_mom_stage(const int32_t* __restrict__ r
     if (a[L - 1] == 0u || b[L - 1] == 0u) {
       mp::zero(r);
     } else {
-      const bool na = mp::to_mag(a), nb_ = mp::to_mag(b);
-      u32 p[2 * L];
-      mp::mul_full(a, b, p);
-      if (na != nb_) mp::neg(p);
-      e = mp::norm<L, 2 * L>(p, re[slot[task] * N + c] + pe[n], r);
+      if (mp::to_mag(b)) mp::neg(a);            // sign of b folded into a
+      u32 acc[2 * L + 2];
+      mp::zero(acc);
+      mp::mac_signed(acc, a, b);
+      e = mp::norm<L, 2 * L + 2>(acc, re[slot[task] * N + c] + pe[n], r);
     }
     const i64 o = woff + row[task] * N + c;
     mp::store(r, sm, sps, o);

@@ -85,11 +85,11 @@ __global__ __launch_bounds__(256) void k_mul(const int32_t* __restrict__ am, con
       mp::zero(r);
       e = B3D_EZ;
     } else {
-      const bool na = mp::to_mag(a), nb = mp::to_mag(b);
-      u32 p[2 * L];
-      mp::mul_full(a, b, p);
-      if (na != nb) mp::neg(p);
-      e = mp::norm<L, 2 * L>(p, ae[i * sa] + be[i * sb], r);
+      if (mp::to_mag(b)) mp::neg(a);            // sign of b folded into a
+      u32 acc[2 * L + 2];
+      mp::zero(acc);
+      mp::mac_signed(acc, a, b);
+      e = mp::norm<L, 2 * L + 2>(acc, ae[i * sa] + be[i * sb], r);
     }
     mp::store(r, cm, cps, i);
     ce[i] = e;

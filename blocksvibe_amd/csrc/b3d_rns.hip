@@ -93,10 +93,11 @@ __global__ __launch_bounds__(256) void k_to_residues(const int32_t* __restrict__
       if (nz) {
         const uint32_t p = primes[tm];
         const uint32_t* pwt = pw + (i64)tm * L;
-        uint64_t s = 0;
+        // sum_q x_q (2^(32 q) mod p): L products < 2^47, exact in FP64 (full rate) for L <= 64
+        double s = 0.0;
 #pragma unroll
-        for (int q = 0; q < L; ++q) s += (uint64_t)x[q] * pwt[q];  // < L 2^47 < 2^53
-        uint32_t r = mod_small(s, p, rp[tm]);
+        for (int q = 0; q < L; ++q) s = fma((double)x[q], (double)pwt[q], s);
+        uint32_t r = mod_small((uint64_t)s, p, rp[tm]);
         if (ng && r) r = p - r;
         int rb = (int)r;
         if (rb > (int)(p >> 1)) rb -= (int)p;  // balanced, |rb| < 2^14

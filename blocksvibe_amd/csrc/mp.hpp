@@ -269,33 +269,30 @@ __device__ __forceinline__ i64 add(const u32 (&a)[L], i64 ea, const u32 (&b)[L],
   return norm<L, L + 3>(wa, tt, r);
 }
 
-// q = floor(|x| * 2^32 / d) for a magnitude x of L limbs and 0 < d < 2^15: L + 1 quotient limbs, computed
-// 16 bits at a time from a float reciprocal estimate with one correction step (the estimate is within
-// +-1 of the true digit: relative error < 3 * 2^-24 on digits < 2^16).
+// q = floor(|x| * 2^32 / d) for a magnitude x of L limbs and 0 < d < 2^15: L + 1 quotient limbs by long
+// division one 32-bit limb at a time in FP64 (full rate on CDNA4): the partial dividend rem 2^32 + limb
+// < 2^47 is exact in a double, the estimate floor(cur * (1/d)) is within +-1 of the true digit (relative
+// error < 2^-52 on a digit < 2^32), and one correction makes it exact; the remainder cur - q d is exact.
 template <int L>
 __device__ __forceinline__ void div_small(const u32 (&x)[L], u32 d, u32 (&q)[L + 1]) {
-  const float rcp = 1.0f / (float)d;
-  u32 rem = 0;
+  const double dd = (double)d;
+  const double rd = 1.0 / dd;
+  double rem = 0.0;
 #pragma unroll
   for (int k = L; k >= 0; --k) {
     const u32 limb = k > 0 ? x[k - 1] : 0u;
-    u32 qq = 0;
-#pragma unroll
-    for (int half = 1; half >= 0; --half) {
-      const u32 cur = (rem << 16) | ((limb >> (16 * half)) & 0xFFFFu);
-      u32 est = (u32)((float)cur * rcp);
-      int r = (int)(cur - __umul24(est, d));
-      if (r < 0) {
-        est -= 1;
-        r += (int)d;
-      } else if (r >= (int)d) {
-        est += 1;
-        r -= (int)d;
-      }
-      rem = (u32)r;
-      qq = (qq << 16) | est;
+    const double cur = rem * 4294967296.0 + (double)limb;
+    double qd = floor(cur * rd);
+    double r = fma(-qd, dd, cur);
+    if (r < 0.0) {
+      qd -= 1.0;
+      r += dd;
+    } else if (r >= dd) {
+      qd += 1.0;
+      r -= dd;
     }
-    q[k] = qq;
+    q[k] = (u32)qd;
+    rem = r;
   }
 }
 
