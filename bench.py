@@ -34,7 +34,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--groups", default="all", help="all | heavy (2j12 = 2j43 = 8 only) | comma list a:b")
-    ap.add_argument("--plain", action="store_true", help="also time the plain per-task path on one group")
+    ap.add_argument("--plain-tasks", type=int, default=4,
+                    help="tasks of the heaviest group run by the plain per-task path (0: skip)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--serve", type=float, default=30.0, help="seconds of the mixed serving stream (0: skip)")
     ap.add_argument("--json-out", default=None)
@@ -86,7 +87,7 @@ class GroupRun:
     def blocks(self):
         return 2 * NQ_PER_SIGN * (TJMAX // 2 + 1)
 
-    def run(self, prof=None, keep=False):
+    def run(self, prof=None, keep=False, timer=None):
         """batched path: per sign, per lambda order one recursion over all structures"""
         results = []
         for sign, p0, der, out, ns, hs in self.subs:
@@ -94,7 +95,7 @@ class GroupRun:
             for n, h in zip(ns, hs):
                 dn = der.for_order(LAMBDA - n)
                 dn.start(h.shape[-1])
-                self.rec.run(h, hook=dn.accumulate, prof=prof)
+                self.rec.run(h, hook=dn.accumulate, prof=prof, timer=timer)
                 nums.append(dn.finish(h))
             r = out.apply_per_order(nums)
             if keep:
@@ -196,42 +197,71 @@ def serving_stream(seconds, seed=7):
             "operators": len(pool)}
 
 
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_cauchy_nmax6_1024bit_pmc.txt")
+
+
+def pmc_traffic(workload):
+    """HBM bytes per k_cauchy launch from the committed rocprofv3 --pmc summary of this workload
+    (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by its reading for a plain copy of known size),
+    or None when there is no summary for this workload"""
+    try:
+        kv = dict(line.split("=", 1) for line in open(PMC_SUMMARY).read().splitlines()
+                  if "=" in line and not line.startswith("#"))
+    except OSError:
+        return None
+    if kv.get("workload", "").strip() != workload:
+        return None
+    return float(kv["traffic_bytes_per_launch"])
+
+
+def roofline(timer, workload):
+    """k_cauchy, the kernel with the largest share of device time: algorithmic bytes per launch (gathered
+    residues, h_inf entries and H entries, each counted once) over its average launch time, both from the
+    timed region (HIP events recorded around every launch on the launch stream)"""
+    import torch
+    torch.cuda.synchronize()
+    ms = [a.elapsed_time(b) for a, b, _ in timer]
+    nbytes = sum(x for _, _, x in timer)
+    avg_s = sum(ms) / len(ms) * 1e-3
+    per_launch = nbytes / len(timer)
+    achieved = per_launch / avg_s / 1e9
+    return {"kernel": "k_cauchy<34>", "bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
+            "frac": achieved / 8000.0, "traffic": pmc_traffic(workload), "launches": len(timer),
+            "avg_launch_ms": avg_s * 1e3, "algorithmic_bytes_per_launch": per_launch}
+
+
 def cpu_baseline(runs, seconds=12.0):
-    """the oracle's bit-level restatement (oracle/mpf_ref: Cauchy terms and pole-step entries at 34 limbs,
-    pure Python on one core), timed on a bounded sample and scaled by the workload's term counts"""
-    import random
-    from oracle import mpf_ref as R
+    """The oracle's C restatement of the recursion arithmetic (oracle/c/mpf_c.c: Cauchy entries and
+    pole-step dot products at 34 limbs, bit-identical to the GPU kernels' spec), timed on all host threads
+    available to this process (at most 16) on a bounded sample, scaled by the workload's exact entry and
+    term counts.  Derivative and output stages are not counted, so the figure is an upper bound on the
+    CPU rate."""
+    from oracle import c_ref as C
     L = 34
-    rng = random.Random(1)
-    nb = R.NB(L)
-    rp = lambda: ((rng.getrandbits(nb - 1) | (1 << (nb - 1))) * rng.choice((-1, 1)), rng.randint(-4, 4) - nb)
-    # Cauchy: groups of 60 columns
-    xs = [rp() for _ in range(60)]
-    ds = [rng.randint(1, 300) * rng.choice((-1, 1)) for _ in range(60)]
-    t0, nterm = time.time(), 0
-    while time.time() - t0 < seconds / 2:
-        R.cauchy(xs, ds, L)
-        nterm += 60
-    t_term = (time.time() - t0) / nterm
-    # pole step entry: 5-term dot products, two per residue entry
-    a = [rp() for _ in range(5)]
-    am, ae = R.align(a)
-    t0, nent = time.time(), 0
-    while time.time() - t0 < seconds / 2:
-        xm, xe = R.align([rp() for _ in range(5)])
-        R.gemm_aligned(am, ae, xm, xe, L)
-        nent += 1
-    t_dot = (time.time() - t0) / nent
+    th = max(1, min(16, len(os.sched_getaffinity(0))))
+    part = seconds / 8
+    # Cauchy: cost = entries * a + terms * b, from entries of 1 and of 4 terms
+    t1 = C.time_unit(0, L, 1, th, part) * 1
+    t4 = C.time_unit(0, L, 4, th, part) * 4
+    b = (t4 - t1) / 3
+    a = t1 - b
+    # pole step: two K-term dots per entry of a task's S x S result, K = the group's S
+    tdot = {k: C.time_unit(1, L, k, th, part) for k in range(1, 6)}
     tot = 0.0
     for g in runs:
-        S2 = g.plan.Smax ** 2
+        pl = g.plan
+        S2 = pl.Smax ** 2
         rhs = 2 * NQ_PER_SIGN * len(g.subs[0][4])
-        tot += rhs * S2 * (g.plan.cauchy_terms() * t_term + 2 * sum(l.T for l in g.plan.levels if l) * t_dot)
+        entries = sum(lv.nH for lv in pl.levels if lv) * S2
+        terms = pl.cauchy_terms() * S2
+        pole = 2 * sum(lv.T for lv in pl.levels if lv) * S2 * tdot[max(1, min(5, pl.Smax))]
+        tot += rhs * (entries * a + terms * b + pole)
     blocks = sum(g.blocks() for g in runs)
-    return {"value": blocks / tot, "unit": "blocks/s", "cores": 1, "kind": "port",
-            "sample": f"oracle mpf_ref Cauchy terms ({t_term * 1e6:.1f} us each) and pole-step dot products "
-                      f"({t_dot * 1e6:.1f} us each) at 34 limbs for {seconds:.0f} s on one core, scaled by the "
-                      f"workload's recursion term counts (derivative and output stages not counted)"}
+    return {"value": blocks / tot, "unit": "blocks/s", "cores": th, "kind": "port",
+            "sample": f"oracle/c/mpf_c.c on {th} threads for {seconds:.0f} s: Cauchy entry {a * 1e9:.0f} ns + "
+                      f"{b * 1e9:.0f} ns per term, 5-term pole dot {tdot[5] * 1e9:.0f} ns (aggregate per unit), "
+                      f"scaled by the workload's recursion entry/term counts; derivative and output stages "
+                      f"not counted"}
 
 
 def main():
@@ -257,12 +287,16 @@ def main():
         torch.cuda.synchronize()
         print(f"# warmup step {w} done", file=sys.stderr, flush=True)
     prof = {}
+    group_s = {}
+    timer = []
     torch.cuda.synchronize()
     t0 = time.time()
     for s in range(args.steps):
         for gi, g in enumerate(runs):
-            g.run()
+            tg = time.time()
+            g.run(timer=timer)
             torch.cuda.synchronize()
+            group_s[(g.tj12, g.tj43)] = group_s.get((g.tj12, g.tj43), 0.0) + time.time() - tg
             print(f"# step {s} group {g.tj12},{g.tj43} done at {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     dt = time.time() - t0
@@ -274,6 +308,21 @@ def main():
            "config": {"workload": "nmax=6 spin-2 externals: 25 operator groups x 22 structures x 6 lambda orders, "
                                   "order 60, kept 14, 1024 bits, xt",
                       "groups": len(runs), "tasks": 22 * len(runs), "blocks_per_step": blocks // args.steps}}
+    res["roofline"] = roofline(timer, res["config"]["workload"])
+    print(f"# roofline: {res['roofline']}", file=sys.stderr, flush=True)
+    if args.plain_tasks > 0:
+        heavy = max(runs, key=lambda g: g.plan.Smax)
+        torch.cuda.synchronize()
+        tp = time.time()
+        n = heavy.run_plain(task_limit=args.plain_tasks)
+        torch.cuda.synchronize()
+        tp = time.time() - tp
+        nb = n * (TJMAX // 2 + 1)
+        batched = heavy.blocks() * args.steps / group_s[(heavy.tj12, heavy.tj43)]
+        res["plain_path"] = {"group": f"{heavy.tj12},{heavy.tj43}", "tasks": n, "blocks_per_s": nb / tp,
+                             "batched_blocks_per_s_same_group": batched, "batched_over_plain": batched / (nb / tp),
+                             "note": "plain: one structure at a time (B = 1 per recursion), same kernels"}
+        print(f"# plain path: {res['plain_path']}", file=sys.stderr, flush=True)
     if args.serve > 0:
         res["serving"] = serving_stream(args.serve)
         print(f"# serving: {res['serving']}", file=sys.stderr, flush=True)

@@ -111,14 +111,22 @@ class Recursion:
                 "ghrow": T((lev.gsrc * O1 + lev.m) * Jh + lev.gtjp // 2, np.int64),
                 "t_hrow": T(lev.t_hrow, np.int64), "t_R": T(lev.t_R, np.int32), "t_qL": T(lev.t_qL, np.int32),
                 "t_slot": T(lev.t_slot, np.int64)})
+        # algorithmic bytes of the level's Cauchy launch per column (structure entry x right-hand side):
+        # every gathered residue read once, one h_inf entry per (source, spin) group, every H entry written once
+        nb = 4 * self.L + 8
+        for lev, lv in zip(plan.levels, self.levels):
+            if lv is not None:
+                g = np.unique(lev.rg)
+                lv["cauchy_bytes"] = nb * (int(lev.ncol[g].sum()) + len(g) + lev.nH)
         live = [lv for lv in self.levels if lv is not None]
         self.maxH = max((lv["nH"] for lv in live), default=1)
         self.maxT = max((lv["T"] for lv in live), default=1)
 
-    def run(self, h_inf, hook=None, prof=None, res=None):
+    def run(self, h_inf, hook=None, prof=None, res=None, timer=None):
         """h_inf: MPF (2, order + 1, Jh, Smax^2, B).  Returns the residue slot buffer MPF
         (nphys + 1, Smax^2, B) (physical slots; the last one is the zero slot).  hook(m, res) runs after each
-        level (e.g. the derivative moments)."""
+        level (e.g. the derivative moments).  timer: a list that receives (start event, end event,
+        algorithmic bytes) of every Cauchy launch, events recorded on the launch stream."""
         plan, L, Sm = self.plan, self.L, self.plan.Smax
         S2 = Sm * Sm
         assert h_inf.shape[:4] == (2, plan.order + 1, plan.Jh, S2), (h_inf.shape, plan.order, plan.Jh, S2)
@@ -138,9 +146,16 @@ class Recursion:
             if lv is None:
                 continue
             t0 = time.time() if prof is not None else None
+            if timer is not None:
+                ev0 = torch.cuda.Event(enable_timing=True)
+                ev0.record()
             _lib.call("b3d_cauchy_level", L, P(res.m), P(res.e), res.numel(), N, P(h.m), P(h.e), h.numel(),
                       P(lv["ghrow"]), P(lv["rg"]), P(lv["rd"]), lv["nH"], P(lv["col"]), P(lv["cold"]),
                       P(lv["ncol"]), lv["Kc"], P(H.m), P(H.e), H.numel(), st)
+            if timer is not None:
+                ev1 = torch.cuda.Event(enable_timing=True)
+                ev1.record()
+                timer.append((ev0, ev1, lv["cauchy_bytes"] * N))
             if prof is not None:
                 torch.cuda.synchronize()
                 t1 = time.time()

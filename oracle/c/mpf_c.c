@@ -1,0 +1,261 @@
+/* C restatement of the oracle's Cauchy-term and pole-step dot-product arithmetic (oracle/mpf_ref.py:
+ * norm, cauchy, gemm_aligned), bit for bit.  TEST INFRASTRUCTURE ONLY: used by tests/ (pinned against
+ * mpf_ref on random and edge-case inputs) and by the CPU baseline leg of bench.py, which times it on all
+ * host threads.  Nothing in the product links it.
+ *
+ * Number format: value m * 2^e, m two's complement over L 32-bit limbs (limb 0 least significant),
+ * normalised bitlen(|m|) = 32 L - 2, zero = (0, OC_EZ).  Arrays here are entry-major: entry i's limbs at
+ * m[i * L .. i * L + L - 1].
+ */
+#include <pthread.h>
+#include <stdint.h>
+#include <string.h>
+#include <time.h>
+
+#define OC_EZ (-(INT64_C(1) << 62))
+#define OC_MAXL 64
+#define OC_MAXW (2 * OC_MAXL + 8)
+
+typedef uint32_t u32;
+typedef uint64_t u64;
+
+static int bitlen(const u32 *a, int n) {
+  for (int k = n - 1; k >= 0; --k)
+    if (a[k]) return 32 * k + 32 - __builtin_clz(a[k]);
+  return 0;
+}
+
+static void negate(u32 *a, int n) {
+  u64 c = 1;
+  for (int k = 0; k < n; ++k) {
+    u64 t = (u64)(u32)~a[k] + c;
+    a[k] = (u32)t;
+    c = t >> 32;
+  }
+}
+
+static int is_neg(const u32 *a, int n) { return (int)(a[n - 1] >> 31); }
+
+/* out[0..no) = a >> s (logical, a has n limbs) */
+static void shr(const u32 *a, int n, int s, u32 *out, int no) {
+  int q = s >> 5, r = s & 31;
+  for (int k = 0; k < no; ++k) {
+    u64 lo = (k + q < n) ? a[k + q] : 0u;
+    u64 hi = (k + q + 1 < n) ? a[k + q + 1] : 0u;
+    out[k] = r ? (u32)((lo >> r) | (hi << (32 - r))) : (u32)lo;
+  }
+}
+
+/* out[0..no) = a << s (a has n limbs) */
+static void shl(const u32 *a, int n, int s, u32 *out, int no) {
+  int q = s >> 5, r = s & 31;
+  for (int k = 0; k < no; ++k) {
+    int j = k - q;
+    u64 lo = (j >= 0 && j < n) ? a[j] : 0u;
+    u64 lo1 = (j - 1 >= 0 && j - 1 < n) ? a[j - 1] : 0u;
+    out[k] = r ? (u32)((lo << r) | (lo1 >> (32 - r))) : (u32)lo;
+  }
+}
+
+/* W (two's complement, n limbs) * 2^e -> normalised L limbs, round half away from zero
+ * (mpf_ref.norm) */
+int64_t oc_norm(const u32 *W, int n, int64_t e, int L, u32 *out) {
+  u32 a[OC_MAXW + 1], r[OC_MAXW + 1];
+  memcpy(a, W, (size_t)n * 4);
+  a[n] = 0;
+  int s = is_neg(a, n);
+  if (s) negate(a, n);
+  int bl = bitlen(a, n);
+  memset(out, 0, (size_t)L * 4);
+  if (!bl) return OC_EZ;
+  int NB = 32 * L - 2, sh = bl - NB;
+  if (sh > 0) {
+    /* add 2^(sh - 1), then shift */
+    int b = sh - 1;
+    u64 c = (u64)1 << (b & 31);
+    for (int k = b >> 5; k <= n && c; ++k) {
+      u64 t = (u64)a[k] + c;
+      a[k] = (u32)t;
+      c = t >> 32;
+    }
+    shr(a, n + 1, sh, r, L + 1);
+    if (bitlen(r, L + 1) > NB) {
+      shr(r, L + 1, 1, out, L);
+      sh += 1;
+    } else {
+      memcpy(out, r, (size_t)L * 4);
+    }
+  } else {
+    shl(a, n, -sh, out, L);
+  }
+  if (s) negate(out, L);
+  return e + sh;
+}
+
+/* acc (n limbs, two's complement) += sign * x (x magnitude, nx limbs) */
+static void acc_add(u32 *acc, int n, const u32 *x, int nx, int sub) {
+  u64 c = sub ? 1 : 0;
+  for (int k = 0; k < n; ++k) {
+    u32 xv = k < nx ? x[k] : 0u;
+    if (sub) xv = ~xv;
+    u64 t = (u64)acc[k] + xv + c;
+    acc[k] = (u32)t;
+    c = t >> 32;
+  }
+}
+
+/* sum_i x_i * 2 / d_i (mpf_ref.cauchy): x entry-major (K x L), d_i == 0 or x_i == 0 contribute nothing */
+int64_t oc_cauchy(const u32 *xm, const int64_t *xe, const int32_t *d, int K, int L, u32 *out) {
+  int64_t E = OC_EZ;
+  int any = 0;
+  for (int i = 0; i < K; ++i) {
+    if (bitlen(xm + (size_t)i * L, L) == 0) continue;
+    if (!any || xe[i] > E) E = xe[i];
+    any = 1;
+  }
+  memset(out, 0, (size_t)L * 4);
+  if (!any) return OC_EZ;
+  u32 acc[OC_MAXL + 3], a[OC_MAXL + 1], q[OC_MAXL + 1], t[OC_MAXL + 1];
+  memset(acc, 0, sizeof(acc));
+  for (int i = 0; i < K; ++i) {
+    const u32 *x = xm + (size_t)i * L;
+    if (d[i] == 0 || bitlen(x, L) == 0) continue;
+    memcpy(a + 1, x, (size_t)L * 4); /* |x| * 2^32 */
+    a[0] = 0;
+    int sx = is_neg(a + 1, L);
+    if (sx) negate(a + 1, L);
+    u64 dd = d[i] < 0 ? (u64)(-(int64_t)d[i]) : (u64)d[i];
+    u64 rem = 0;
+    for (int k = L; k >= 0; --k) {
+      u64 cur = (rem << 32) | a[k];
+      q[k] = (u32)(cur / dd);
+      rem = cur % dd;
+    }
+    int64_t sh = E - xe[i];
+    if (sh >= 32 * (L + 1)) continue;
+    shr(q, L + 1, (int)sh, t, L + 1);
+    acc_add(acc, L + 3, t, L + 1, sx != (d[i] < 0));
+  }
+  return oc_norm(acc, L + 3, E - 31, L, out);
+}
+
+/* one pole-step entry from pre-aligned operands (mpf_ref.gemm_aligned): exact sum of K products of
+ * L-limb signed mantissas, one rounding */
+int64_t oc_dot_aligned(const u32 *am, int64_t ae, const u32 *bm, int64_t be, int K, int L, u32 *out) {
+  memset(out, 0, (size_t)L * 4);
+  if (ae == OC_EZ || be == OC_EZ) return OC_EZ;
+  u32 acc[OC_MAXW], x[OC_MAXL], y[OC_MAXL], p[2 * OC_MAXL];
+  memset(acc, 0, sizeof(acc));
+  for (int i = 0; i < K; ++i) {
+    memcpy(x, am + (size_t)i * L, (size_t)L * 4);
+    memcpy(y, bm + (size_t)i * L, (size_t)L * 4);
+    int sx = is_neg(x, L), sy = is_neg(y, L);
+    if (sx) negate(x, L);
+    if (sy) negate(y, L);
+    memset(p, 0, (size_t)2 * L * 4);
+    for (int j = 0; j < L; ++j) {
+      if (!x[j]) continue;
+      u64 c = 0;
+      for (int k = 0; k < L; ++k) {
+        u64 t = (u64)x[j] * y[k] + p[j + k] + c;
+        p[j + k] = (u32)t;
+        c = t >> 32;
+      }
+      p[j + L] = (u32)c;
+    }
+    acc_add(acc, 2 * L + 2, p, 2 * L, sx != sy);
+  }
+  return oc_norm(acc, 2 * L + 2, ae + be, L, out);
+}
+
+/* ------------------------------------------------------------------ multithreaded timing (CPU baseline) */
+
+typedef struct {
+  int L, K, kind;
+  double seconds;
+  u64 seed;
+  double units, elapsed;
+} job_t;
+
+static u64 xs64(u64 *s) {
+  u64 x = *s;
+  x ^= x << 13;
+  x ^= x >> 7;
+  x ^= x << 17;
+  return *s = x;
+}
+
+static double now(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+/* random normalised mantissa with exponent spread 0..3 below the top */
+static void rnd_mant(u32 *m, int L, u64 *s) {
+  for (int k = 0; k < L; ++k) m[k] = (u32)xs64(s);
+  m[L - 1] &= 0x1FFFFFFFu;
+  m[L - 1] |= 0x20000000u;
+  if (xs64(s) & 1) negate(m, L);
+}
+
+static void *worker(void *arg) {
+  job_t *j = (job_t *)arg;
+  int L = j->L, K = j->K;
+  u32 xm[64 * OC_MAXL], bm[64 * OC_MAXL], out[OC_MAXL];
+  int64_t xe[64];
+  int32_t d[64];
+  u64 s = j->seed | 1;
+  for (int i = 0; i < K; ++i) {
+    rnd_mant(xm + (size_t)i * L, L, &s);
+    rnd_mant(bm + (size_t)i * L, L, &s);
+    xe[i] = -(int64_t)(32 * L) + (int64_t)(xs64(&s) % 5);
+    d[i] = (int32_t)(xs64(&s) % 600) - 300;
+    if (d[i] == 0) d[i] = 1;
+  }
+  double t0 = now(), t = t0, units = 0;
+  volatile u32 sink = 0;
+  while (t - t0 < j->seconds) {
+    for (int r = 0; r < 64; ++r) {
+      if (j->kind == 0) {
+        oc_cauchy(xm, xe, d, K, L, out);
+        units += K;
+      } else {
+        oc_dot_aligned(xm, -100, bm, -100, K, L, out);
+        units += 1;
+      }
+      sink ^= out[0];
+      xm[r % K * L] ^= out[1] | 1u;
+    }
+    t = now();
+  }
+  j->units = units;
+  j->elapsed = t - t0;
+  return 0;
+}
+
+/* per-unit wall time on nthreads threads running concurrently: kind 0 = one Cauchy term (groups of K
+ * terms), kind 1 = one K-term pole-step dot product.  Returns seconds per unit for the whole machine
+ * (total elapsed / total units across threads, i.e. the aggregate throughput's inverse). */
+double oc_time(int kind, int L, int K, int nthreads, double seconds) {
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > 256) nthreads = 256;
+  if (K > 64) K = 64;
+  pthread_t th[256];
+  job_t jobs[256];
+  for (int i = 0; i < nthreads; ++i) {
+    jobs[i].L = L;
+    jobs[i].K = K;
+    jobs[i].kind = kind;
+    jobs[i].seconds = seconds;
+    jobs[i].seed = 0x9E3779B97F4A7C15ull * (u64)(i + 1);
+    pthread_create(&th[i], 0, worker, &jobs[i]);
+  }
+  double units = 0, el = 0;
+  for (int i = 0; i < nthreads; ++i) {
+    pthread_join(th[i], 0);
+    units += jobs[i].units;
+    if (jobs[i].elapsed > el) el = jobs[i].elapsed;
+  }
+  return units > 0 ? el / units : 0.0;
+}
