@@ -36,7 +36,7 @@ __global__ __launch_bounds__(256) void k_mom_stage(const int32_t* __restrict__ r
     if (a[L - 1] == 0u || b[L - 1] == 0u) {
       mp::zero(r);
     } else {
-      if (mp::to_mag(b)) mp::neg(a);            // sign of b folded into a
+      mp::cneg(a, mp::to_mag(b));               // sign of b folded into a
       u32 acc[2 * L + 2];
       mp::zero(acc);
       mp::mac_signed(acc, a, b);

@@ -85,7 +85,7 @@ __global__ __launch_bounds__(256) void k_mul(const int32_t* __restrict__ am, con
       mp::zero(r);
       e = B3D_EZ;
     } else {
-      if (mp::to_mag(b)) mp::neg(a);            // sign of b folded into a
+      mp::cneg(a, mp::to_mag(b));               // sign of b folded into a
       u32 acc[2 * L + 2];
       mp::zero(acc);
       mp::mac_signed(acc, a, b);
@@ -116,10 +116,10 @@ __global__ __launch_bounds__(256) void k_align_rows(const int32_t* __restrict__ 
     const i64 idx = r * rs + k * ks;
     mp::load(a, m, ps, idx);
     if (a[L - 1] != 0u) {
-      const bool na = mp::to_mag(a);
+      const u32 na = mp::to_mag(a);
       const i64 sh = E - e[idx];
       mp::shr(a, sh > 32 * L ? 32 * L : (int)sh);
-      if (na) mp::neg(a);
+      mp::cneg(a, na);
     }
     mp::store(a, om, ops, t);
     if (k == 0) oe[r] = E;

@@ -83,7 +83,7 @@ __global__ __launch_bounds__(256) void k_to_residues(const int32_t* __restrict__
       mp::load(x, m, ps, idx);
       if (x[L - 1] != 0u) {
         nz = true;
-        ng = mp::to_mag(x);
+        ng = mp::to_mag(x) != 0u;
         const i64 sh = Eline[b * estride + line] - e[idx];
         mp::shr(x, sh > 32 * L ? 32 * L : (int)sh);
       }

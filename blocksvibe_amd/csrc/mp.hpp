@@ -2,8 +2,11 @@
 //
 // Every array index below is a compile-time constant after unrolling (shifts by a run-time amount are
 // barrel shifters built from selects), so the arrays stay in registers instead of falling back to scratch
-// memory.  Number format: value = m * 2^e, m two's complement over L 32-bit limbs (limb 0 least
-// significant), normalised bitlen(|m|) = NB = 32 L - 2, zero = (0, EZ).
+// memory.  Carry chains use the hardware carry flag (__builtin_addc / __builtin_subc -> v_add_co /
+// v_addc_co, one instruction per limb); a barrel stage that no lane of the wave needs is skipped with a
+// wave-uniform branch (ballot), and sign handling is branch-free (xor with a sign mask, carry in).
+// Number format: value = m * 2^e, m two's complement over L 32-bit limbs (limb 0 least significant),
+// normalised bitlen(|m|) = NB = 32 L - 2, zero = (0, EZ).
 #pragma once
 #ifndef B3D_HOST_CHECK
 #include <hip/hip_runtime.h>
@@ -16,7 +19,32 @@ typedef int64_t i64;
 
 #define B3D_EZ (-(1LL << 62))
 
+#ifdef B3D_HOST_CHECK
+#define B3D_ANY(p) (p)
+static inline u32 b3d_alignbit(u32 hi, u32 lo, u32 r) {
+  r &= 31u;
+  return r ? (lo >> r) | (hi << (32u - r)) : lo;
+}
+#else
+#define B3D_ANY(p) (__builtin_amdgcn_ballot_w64(p) != 0)
+#define b3d_alignbit(hi, lo, r) __builtin_amdgcn_alignbit((hi), (lo), (r))
+#endif
+
 namespace mp {
+
+__device__ __forceinline__ u32 addc(u32 a, u32 b, u32& c) {
+  unsigned co;
+  const u32 r = __builtin_addc(a, b, c, &co);
+  c = co;
+  return r;
+}
+
+__device__ __forceinline__ u32 subb(u32 a, u32 b, u32& br) {
+  unsigned bo;
+  const u32 r = __builtin_subc(a, b, br, &bo);
+  br = bo;
+  return r;
+}
 
 template <int N>
 __device__ __forceinline__ void zero(u32 (&a)[N]) {
@@ -24,16 +52,31 @@ __device__ __forceinline__ void zero(u32 (&a)[N]) {
   for (int k = 0; k < N; ++k) a[k] = 0u;
 }
 
-// two's complement negation in place
+// all-ones if a (two's complement) is negative, else 0
+template <int N>
+__device__ __forceinline__ u32 sign_mask(const u32 (&a)[N]) {
+  return 0u - (a[N - 1] >> 31);
+}
+
+// a = (a ^ m) + (m & 1): two's complement negation for m = ~0, identity for m = 0
+template <int N>
+__device__ __forceinline__ void cneg(u32 (&a)[N], u32 m) {
+  u32 c = m & 1u;
+#pragma unroll
+  for (int k = 0; k < N; ++k) a[k] = addc(a[k] ^ m, 0u, c);
+}
+
 template <int N>
 __device__ __forceinline__ void neg(u32 (&a)[N]) {
-  u64 c = 1;
-#pragma unroll
-  for (int k = 0; k < N; ++k) {
-    u64 v = (u64)(~a[k]) + c;
-    a[k] = (u32)v;
-    c = v >> 32;
-  }
+  cneg(a, 0xFFFFFFFFu);
+}
+
+// sign-magnitude from two's complement (in place); returns the sign mask (~0 if it was negative)
+template <int N>
+__device__ __forceinline__ u32 to_mag(u32 (&a)[N]) {
+  const u32 m = sign_mask(a);
+  cneg(a, m);
+  return m;
 }
 
 template <int N>
@@ -49,66 +92,68 @@ __device__ __forceinline__ int bitlen(const u32 (&a)[N]) {
   int bl = 0;
 #pragma unroll
   for (int k = 0; k < N; ++k)
-    if (a[k]) bl = 32 * k + 32 - __clz(a[k]);
+    if (a[k]) bl = 32 * k + 32 - __builtin_clz(a[k]);
   return bl;
 }
 
-// a >>= 32*q limbs (q >= 0), zero fill
-template <int N>
-__device__ __forceinline__ void shr_limbs(u32 (&a)[N], int q) {
-  if (q >= N) {
-    zero(a);
-    return;
-  }
+// out[0..NO) = in >> s (logical, NI input limbs, s >= 0).  Barrel stages from the largest: after the stage
+// of 2^b limbs, only the NO + 2^b lowest limbs can still reach the output.  Stages no lane needs are
+// skipped; the bit part is one funnel shift per output limb.
+template <int NO, int NI>
+__device__ __forceinline__ void shr_to(u32 (&out)[NO], u32 (&a)[NI], int s) {
+  int q = s >> 5;
+  if (q > NI) q = NI;
+  constexpr int TOP = (NI >= 64) ? 6 : (NI >= 32) ? 5 : (NI >= 16) ? 4 : (NI >= 8) ? 3 : (NI >= 4) ? 2 : (NI >= 2) ? 1 : 0;
 #pragma unroll
-  for (int b = 0; (1 << b) < N; ++b) {
+  for (int b = TOP; b >= 0; --b) {
     const int st = 1 << b;
     const bool on = (q >> b) & 1;
+    if (B3D_ANY(on)) {
 #pragma unroll
-    for (int k = 0; k < N; ++k) a[k] = on ? (k + st < N ? a[k + st] : 0u) : a[k];
-  }
-}
-
-// a <<= 32*q limbs (q >= 0), zero fill, bits shifted out are lost
-template <int N>
-__device__ __forceinline__ void shl_limbs(u32 (&a)[N], int q) {
-  if (q >= N) {
-    zero(a);
-    return;
-  }
-#pragma unroll
-  for (int b = 0; (1 << b) < N; ++b) {
-    const int st = 1 << b;
-    const bool on = (q >> b) & 1;
-#pragma unroll
-    for (int k = N - 1; k >= 0; --k) a[k] = on ? (k - st >= 0 ? a[k - st] : 0u) : a[k];
-  }
-}
-
-// logical right shift by s >= 0 bits (unsigned magnitude), truncating
-template <int N>
-__device__ __forceinline__ void shr(u32 (&a)[N], int s) {
-  shr_limbs(a, s >> 5);
-  const u32 r = (u32)(s & 31);
-  if (r) {
-#pragma unroll
-    for (int k = 0; k < N; ++k) {
-      u32 hi = (k + 1 < N) ? a[k + 1] : 0u;
-      a[k] = (a[k] >> r) | (hi << (32u - r));
+      for (int k = 0; k < NI; ++k)
+        if (k < NO + st) a[k] = on ? (k + st < NI ? a[k + st] : 0u) : a[k];
     }
   }
+  // q == NI (the clamp) has already moved zeros into every limb the output reads
+  const u32 r = (u32)(s & 31);
+#pragma unroll
+  for (int k = 0; k < NO; ++k) {
+    const u32 lo = k < NI ? a[k] : 0u;
+    const u32 hi = k + 1 < NI ? a[k + 1] : 0u;
+    out[k] = b3d_alignbit(hi, lo, r);
+  }
 }
 
-// left shift by s >= 0 bits
+// logical right shift in place by s >= 0 bits (truncating)
+template <int N>
+__device__ __forceinline__ void shr(u32 (&a)[N], int s) {
+  u32 t[N];
+  shr_to(t, a, s);
+#pragma unroll
+  for (int k = 0; k < N; ++k) a[k] = t[k];
+}
+
+// left shift in place by s >= 0 bits (bits shifted out are lost)
 template <int N>
 __device__ __forceinline__ void shl(u32 (&a)[N], int s) {
-  shl_limbs(a, s >> 5);
+  int q = s >> 5;
+  if (q > N) q = N;
+  constexpr int TOP = (N >= 64) ? 6 : (N >= 32) ? 5 : (N >= 16) ? 4 : (N >= 8) ? 3 : (N >= 4) ? 2 : (N >= 2) ? 1 : 0;
+#pragma unroll
+  for (int b = TOP; b >= 0; --b) {
+    const int st = 1 << b;
+    const bool on = (q >> b) & 1;
+    if (B3D_ANY(on)) {
+#pragma unroll
+      for (int k = N - 1; k >= 0; --k) a[k] = on ? (k - st >= 0 ? a[k - st] : 0u) : a[k];
+    }
+  }
   const u32 r = (u32)(s & 31);
-  if (r) {
+  if (B3D_ANY(r != 0u)) {
 #pragma unroll
     for (int k = N - 1; k >= 0; --k) {
-      u32 lo = (k > 0) ? a[k - 1] : 0u;
-      a[k] = (a[k] << r) | (lo >> (32u - r));
+      const u32 lo = k > 0 ? a[k - 1] : 0u;
+      a[k] = r ? b3d_alignbit(a[k], lo, 32u - r) : a[k];
     }
   }
 }
@@ -116,13 +161,9 @@ __device__ __forceinline__ void shl(u32 (&a)[N], int s) {
 // acc += x (N >= M limbs, x zero-extended), two's complement wrap
 template <int N, int M>
 __device__ __forceinline__ void add_into(u32 (&acc)[N], const u32 (&x)[M]) {
-  u64 c = 0;
+  u32 c = 0;
 #pragma unroll
-  for (int k = 0; k < N; ++k) {
-    u64 v = (u64)acc[k] + (k < M ? x[k] : 0u) + c;
-    acc[k] = (u32)v;
-    c = v >> 32;
-  }
+  for (int k = 0; k < N; ++k) acc[k] = addc(acc[k], k < M ? x[k] : 0u, c);
 }
 
 // acc -= x (N >= M limbs)
@@ -130,29 +171,33 @@ template <int N, int M>
 __device__ __forceinline__ void sub_from(u32 (&acc)[N], const u32 (&x)[M]) {
   u32 b = 0;
 #pragma unroll
-  for (int k = 0; k < N; ++k) {
-    u64 v = (u64)acc[k] - (u64)(k < M ? x[k] : 0u) - (u64)b;
-    acc[k] = (u32)v;
-    b = (u32)(v >> 63);
-  }
+  for (int k = 0; k < N; ++k) acc[k] = subb(acc[k], k < M ? x[k] : 0u, b);
 }
 
-// p = a * b (magnitudes), p has NA + NB limbs
+// acc += x if m == 0, acc -= x if m == ~0 (x a zero-extended magnitude of M <= N limbs)
+template <int N, int M>
+__device__ __forceinline__ void cadd(u32 (&acc)[N], const u32 (&x)[M], u32 m) {
+  u32 c = m & 1u;
+#pragma unroll
+  for (int k = 0; k < N; ++k) acc[k] = addc(acc[k], (k < M ? x[k] : 0u) ^ m, c);
+}
+
+// p = a * b (magnitudes), p has NA + NBL limbs
 template <int NA, int NBL>
 __device__ __forceinline__ void mul_full(const u32 (&a)[NA], const u32 (&b)[NBL], u32 (&p)[NA + NBL]) {
 #pragma unroll
   for (int k = 0; k < NA + NBL; ++k) p[k] = 0u;
 #pragma unroll
   for (int x = 0; x < NA; ++x) {
-    const u32 ax = a[x];
-    u64 c = 0;
+    u32 c = 0;
 #pragma unroll
     for (int y = 0; y < NBL; ++y) {
-      u64 v = (u64)ax * b[y] + (u64)p[x + y] + c;
-      p[x + y] = (u32)v;
-      c = v >> 32;
+      const u64 v = (u64)a[x] * b[y] + (u64)p[x + y];   // < 2^64: no overflow
+      u32 cc = 0;
+      p[x + y] = addc((u32)v, c, cc);
+      c = (u32)(v >> 32) + cc;                         // <= 2^32 - 1
     }
-    p[x + NBL] = (u32)c;
+    p[x + NBL] = c;
   }
 }
 
@@ -162,8 +207,7 @@ __device__ __forceinline__ void mul_full(const u32 (&a)[NA], const u32 (&b)[NBL]
 template <int L, int NW>
 __device__ __forceinline__ i64 norm(u32 (&acc)[NW], i64 e, u32 (&m)[L]) {
   const int NBITS = 32 * L - 2;
-  const bool neg_ = (acc[NW - 1] >> 31) != 0u;
-  if (neg_) neg(acc);
+  const u32 sm = to_mag(acc);
   const int bl = bitlen(acc);
   if (bl == 0) {
     zero(m);
@@ -171,32 +215,27 @@ __device__ __forceinline__ i64 norm(u32 (&acc)[NW], i64 e, u32 (&m)[L]) {
   }
   int s = bl - NBITS;
   if (s > 0) {
-    // add 2^(s-1), then shift right by s
-    const int pos = s - 1;
-    const int pq = pos >> 5;
-    const u32 pb = 1u << (pos & 31);
-    u64 c = 0;
+    // floor(|W| / 2^(s-1)) has NB + 1 bits; + 1 then >> 1 rounds half away from zero
+    u32 t[L];
+    shr_to(t, acc, s - 1);
+    u32 c = 1u;
 #pragma unroll
-    for (int k = 0; k < NW; ++k) {
-      u64 v = (u64)acc[k] + (k == pq ? pb : 0u) + c;
-      acc[k] = (u32)v;
-      c = v >> 32;
-    }
-    shr(acc, s);
-    // a carry out of the rounding gives exactly 2^NB: one more bit down
-    const bool over = (acc[L - 1] >> 30) != 0u || (NW > L && acc[L] != 0u);
-    if (over) {
-      shr(acc, 1);
-      s += 1;
-    }
+    for (int k = 0; k < L; ++k) t[k] = addc(t[k], 0u, c);
 #pragma unroll
-    for (int k = 0; k < L; ++k) m[k] = acc[k];
+    for (int k = 0; k < L; ++k) m[k] = b3d_alignbit(k + 1 < L ? t[k + 1] : 0u, t[k], 1u);
+    // a carry out of the rounding gives exactly 2^NB: 2^(NB-1) one exponent up
+    const bool over = (m[L - 1] >> 30) != 0u;
+    if (B3D_ANY(over)) {
+#pragma unroll
+      for (int k = 0; k < L; ++k) m[k] = over ? (k == L - 1 ? (1u << 29) : 0u) : m[k];
+    }
+    s += over ? 1 : 0;
   } else {
 #pragma unroll
     for (int k = 0; k < L; ++k) m[k] = k < NW ? acc[k] : 0u;
-    if (s < 0) shl(m, -s);
+    shl(m, -s);
   }
-  if (neg_) neg(m);
+  cneg(m, sm);
   return e + s;
 }
 
@@ -211,14 +250,6 @@ template <int N>
 __device__ __forceinline__ void store(const u32 (&a)[N], int32_t* __restrict__ p, i64 ps, i64 idx) {
 #pragma unroll
   for (int k = 0; k < N; ++k) p[k * ps + idx] = (int32_t)a[k];
-}
-
-// sign-magnitude from two's complement (in place); returns true if negative
-template <int N>
-__device__ __forceinline__ bool to_mag(u32 (&a)[N]) {
-  const bool n = (a[N - 1] >> 31) != 0u;
-  if (n) neg(a);
-  return n;
 }
 
 // a + b for normalised operands: both put on the grid 2^(max(ea,eb) - 32), the smaller truncated, exact
@@ -241,101 +272,91 @@ __device__ __forceinline__ i64 add(const u32 (&a)[L], i64 ea, const u32 (&b)[L],
     return ea;
   }
   const i64 tt = (ea > eb ? ea : eb) - 32;
-  // magnitudes (negated in L limbs, then zero-extended)
   u32 ma[L], mb[L];
 #pragma unroll
   for (int k = 0; k < L; ++k) {
     ma[k] = a[k];
     mb[k] = b[k];
   }
-  const bool na = to_mag(ma), nb = to_mag(mb);
-  u32 wa[L + 3], wb[L + 3];
+  const u32 sa = to_mag(ma), sb = to_mag(mb);
+  u32 wa[L + 1], wb[L + 1];
 #pragma unroll
-  for (int k = 0; k < L + 3; ++k) {
+  for (int k = 0; k < L + 1; ++k) {
     wa[k] = k < L ? ma[k] : 0u;
     wb[k] = k < L ? mb[k] : 0u;
   }
-  {
-    const i64 da = ea - tt;  // <= 32
-    if (da >= 0) shl(wa, (int)da);
-    else shr(wa, (-da) > 32 * (L + 3) ? 32 * (L + 3) : (int)(-da));
-    const i64 db = eb - tt;
-    if (db >= 0) shl(wb, (int)db);
-    else shr(wb, (-db) > 32 * (L + 3) ? 32 * (L + 3) : (int)(-db));
-  }
-  if (na) neg(wa);
-  if (nb) neg(wb);
-  add_into(wa, wb);
-  return norm<L, L + 3>(wa, tt, r);
+  // the larger exponent is shifted up by exactly 32 (one limb), the other down by its distance
+  const i64 da = ea - tt, db = eb - tt;   // <= 32, one of them == 32
+  if (da >= 0) shl(wa, (int)da);
+  else shr(wa, (-da) > 32 * (L + 1) ? 32 * (L + 1) : (int)(-da));
+  if (db >= 0) shl(wb, (int)db);
+  else shr(wb, (-db) > 32 * (L + 1) ? 32 * (L + 1) : (int)(-db));
+  u32 acc[L + 3];
+  zero(acc);
+  cadd(acc, wa, sa);
+  cadd(acc, wb, sb);
+  return norm<L, L + 3>(acc, tt, r);
 }
 
 // q = floor(|x| * 2^32 / d) for a magnitude x of L limbs and 0 < d < 2^15: L + 1 quotient limbs by long
-// division one 32-bit limb at a time in FP64 (full rate on CDNA4): the partial dividend rem 2^32 + limb
-// < 2^47 is exact in a double, the estimate floor(cur * (1/d)) is within +-1 of the true digit (relative
-// error < 2^-52 on a digit < 2^32), and one correction makes it exact; the remainder cur - q d is exact.
+// division one 32-bit limb at a time in FP64 (full rate on CDNA4).  The partial dividend
+// cur = rem 2^32 + limb < 2^47 is exact; fma(cur, 1/d, 2^-19) is within 2^-20 of cur/d + 2^-19 (rounding
+// of 1/d and of the fma, each < 2^-21 on a value < 2^32), and the fractional part of cur/d is at most
+// 1 - 1/d < 1 - 2^-15, so its floor is the exact digit; the remainder cur - q d is exact.
 template <int L>
 __device__ __forceinline__ void div_small(const u32 (&x)[L], u32 d, u32 (&q)[L + 1]) {
   const double dd = (double)d;
   const double rd = 1.0 / dd;
+  const double bias = 1.0 / 524288.0;   // 2^-19
   double rem = 0.0;
 #pragma unroll
   for (int k = L; k >= 0; --k) {
     const u32 limb = k > 0 ? x[k - 1] : 0u;
-    const double cur = rem * 4294967296.0 + (double)limb;
-    double qd = floor(cur * rd);
-    double r = fma(-qd, dd, cur);
-    if (r < 0.0) {
-      qd -= 1.0;
-      r += dd;
-    } else if (r >= dd) {
-      qd += 1.0;
-      r -= dd;
-    }
+    const double cur = fma(rem, 4294967296.0, (double)limb);
+    const double qd = floor(fma(cur, rd, bias));
+    rem = fma(-qd, dd, cur);
     q[k] = (u32)qd;
-    rem = r;
   }
 }
 
-
 // acc += a * x, a an L-limb two's complement value, x an L-limb magnitude, acc NW >= 2L + 1 limbs
 // (wrapping).  Product scanning: the column sums of the unsigned product bits(a) * x are accumulated in
-// 96 bits and folded into acc one limb at a time (no product array, no carry propagation per row); a
-// negative a contributes bits(a) = a + 2^(32L), corrected by subtracting x at limb offset L.
+// 96 bits (t0 + a_i x_j < 2^64 is one v_mad_u64_u32, the high half goes into t1:t2 with the carry flag)
+// and folded into acc one limb at a time; a negative a contributes bits(a) = a + 2^(32L), corrected by
+// subtracting x at limb offset L.
 template <int L, int NW>
 __device__ __forceinline__ void mac_signed(u32 (&acc)[NW], const u32 (&a)[L], const u32 (&x)[L]) {
-  u64 t01 = 0;
-  u32 t2 = 0;
+  u32 t0 = 0, t1 = 0, t2 = 0;
 #pragma unroll
   for (int s = 0; s < 2 * L - 1; ++s) {
 #pragma unroll
     for (int i = (s - L + 1 > 0 ? s - L + 1 : 0); i <= (s < L - 1 ? s : L - 1); ++i) {
-      const u64 prod = (u64)a[i] * x[s - i];
-      const u64 n = t01 + prod;
-      t2 += (n < prod) ? 1u : 0u;
-      t01 = n;
+      const u64 v = (u64)a[i] * x[s - i] + (u64)t0;
+      t0 = (u32)v;
+      u32 c = 0;
+      t1 = addc(t1, (u32)(v >> 32), c);
+      t2 = addc(t2, 0u, c);
     }
-    const u64 n = t01 + (u64)acc[s];
-    t2 += (n < t01) ? 1u : 0u;
-    acc[s] = (u32)n;
-    t01 = (n >> 32) | ((u64)t2 << 32);
+    u32 c = 0;
+    acc[s] = addc(acc[s], t0, c);
+    t0 = addc(t1, 0u, c);
+    t1 = t2 + c;
     t2 = 0;
   }
-  u64 c = t01;
+  {
+    u32 c = 0;
+    acc[2 * L - 1] = addc(acc[2 * L - 1], t0, c);
+    u32 hi = t1;
 #pragma unroll
-  for (int s = 2 * L - 1; s < NW; ++s) {
-    const u64 n = (u64)acc[s] + (c & 0xFFFFFFFFull);
-    acc[s] = (u32)n;
-    c = (c >> 32) + (n >> 32);
-  }
-  if (a[L - 1] >> 31) {                      // acc -= x * 2^(32 L)
-    u32 b = 0;
-#pragma unroll
-    for (int s = L; s < NW; ++s) {
-      const u64 v = (u64)acc[s] - (u64)(s - L < L ? x[s - L] : 0u) - (u64)b;
-      acc[s] = (u32)v;
-      b = (u32)(v >> 63);
+    for (int s = 2 * L; s < NW; ++s) {
+      acc[s] = addc(acc[s], hi, c);
+      hi = 0;
     }
   }
+  const u32 m = sign_mask(a);                 // acc -= x * 2^(32 L) if a < 0
+  u32 b = 0;
+#pragma unroll
+  for (int s = L; s < NW; ++s) acc[s] = subb(acc[s], (s - L < L ? x[s - L] : 0u) & m, b);
 }
 
 // out = norm(sum_j A[j] * X[j]) over K = S terms: A pre-aligned (mantissas a_j, exponent ea), X entries
@@ -363,23 +384,24 @@ __device__ __forceinline__ i64 small_dot(const int32_t* __restrict__ am, i64 aps
     u32 x[L];
     mp::load(x, xm, xps, idx);
     if (x[L - 1] == 0u) continue;
-    const bool nx = mp::to_mag(x);
+    const u32 sx = mp::to_mag(x);
     const i64 sh = E - xe[idx];
     if (sh >= 32 * L) continue;
     mp::shr(x, (int)sh);
     u32 a[L];
     mp::load(a, am, aps, abase + j);
-    if (nx) mp::neg(a);                     // sign of x folded into a (|a| < 2^(32L-2): no overflow)
+    mp::cneg(a, sx);                        // sign of x folded into a (|a| < 2^(32L-2): no overflow)
     mac_signed(acc, a, x);
   }
   return mp::norm<L, 2 * L + 2>(acc, ea + E, out);
 }
 
 // One Cauchy-step entry: h + sum_i x_i * 2 / (D - D_i) for column n of a group whose prime poles are the
-// slots colg[i] (i < nc) at doubled positions coldg[i].  The x_i (slot buffer rm/re, plane stride rps,
-// N columns per slot) are aligned to their largest exponent E; term i = sign * floor(|x_i| 2^32 / |d|)
-// >> (E - e_i) on the grid 2^(E - 31); exact sum in L + 3 limbs, one rounding, then add(h) with the
-// h_inf entry hidx of hm/he.  Returns the exponent, mantissa in r.
+// slots colg[i] (i < nc) at doubled positions coldg[i], h the h_inf entry hidx of hm/he, with one rounding.
+// E = largest exponent of the nonzero x_i (slot buffer rm/re, plane stride rps, N columns per slot);
+// grid G = max(E - 31, e_h - 32) (either side alone when the other is absent); term i =
+// sign * floor(|x_i| 2^32 / |d|) >> (G + 31 - e_i) and h truncated onto 2^G; exact sum in L + 3 limbs,
+// one rounding.  Returns the exponent, mantissa in r.
 template <int L>
 __device__ __forceinline__ i64 cauchy_entry(const int32_t* __restrict__ rm, const i64* __restrict__ re, i64 rps,
                                             i64 N, i64 n, const int64_t* __restrict__ colg,
@@ -394,6 +416,15 @@ __device__ __forceinline__ i64 cauchy_entry(const int32_t* __restrict__ rm, cons
       E = ei > E ? ei : E;
     }
   }
+  u32 hv[L];
+  load(hv, hm, hps, hidx);
+  const bool hz = hv[L - 1] == 0u;
+  const i64 eh = he[hidx];
+  if (E == B3D_EZ && hz) {
+    zero(r);
+    return B3D_EZ;
+  }
+  const i64 G = E == B3D_EZ ? eh - 32 : (hz ? E - 31 : ((E - 31) > (eh - 32) ? (E - 31) : (eh - 32)));
   u32 acc[L + 3];
   zero(acc);
   if (E != B3D_EZ) {
@@ -404,25 +435,29 @@ __device__ __forceinline__ i64 cauchy_entry(const int32_t* __restrict__ rm, cons
       u32 x[L];
       load(x, rm, rps, idx);
       if (x[L - 1] == 0u) continue;
-      bool ng = to_mag(x);
-      if (d < 0) {
-        ng = !ng;
-        d = -d;
-      }
-      const i64 sh = E - re[idx];
+      const i64 sh = G + 31 - re[idx];
       if (sh >= 32 * (L + 1)) continue;
+      u32 sg = to_mag(x);
+      const u32 sd = 0u - (u32)(d < 0);
+      sg ^= sd;
+      d = d < 0 ? -d : d;
       u32 q[L + 1];
       div_small<L>(x, (u32)d, q);
       shr(q, (int)sh);
-      if (ng) sub_from(acc, q);
-      else add_into(acc, q);
+      cadd(acc, q, sg);
     }
   }
-  u32 c[L];
-  const i64 ec = norm<L, L + 3>(acc, E - 31, c);
-  u32 hv[L];
-  load(hv, hm, hps, hidx);
-  return add<L>(c, ec, hv, he[hidx], r);
+  if (!hz) {
+    const u32 sh_ = to_mag(hv);
+    u32 y[L + 1];
+#pragma unroll
+    for (int k = 0; k < L + 1; ++k) y[k] = k < L ? hv[k] : 0u;
+    const i64 t = eh - G;   // <= 32
+    if (t >= 0) shl(y, (int)t);
+    else shr(y, (-t) > 32 * (L + 1) ? 32 * (L + 1) : (int)(-t));
+    cadd(acc, y, sh_);
+  }
+  return norm<L, L + 3>(acc, G, r);
 }
 
 // One windowed moment: sum_w y_w n_w^k for the staged entries y_w = stage[w, row, c] (w < W), n_w = n0 + w,
@@ -454,7 +489,7 @@ __device__ __forceinline__ i64 mom_window(const int32_t* __restrict__ sm, const 
     u32 y[L];
     load(y, sm, sps, o);
     if (y[L - 1] == 0u) continue;
-    const bool ny = to_mag(y);
+    const u32 sy = to_mag(y);
     const i64 ni = (i64)(n0 + w) * R1 + k;
     u32 nk[KLM];
 #pragma unroll
@@ -471,8 +506,7 @@ __device__ __forceinline__ i64 mom_window(const int32_t* __restrict__ sm, const 
     u32 q[L + 2];
 #pragma unroll
     for (int z = 0; z < L + 2; ++z) q[z] = z < L + KLM ? p[z] : 0u;
-    if (ny) sub_from(acc, q);
-    else add_into(acc, q);
+    cadd(acc, q, sy);
   }
   return norm<L, L + 3>(acc, T, r);
 }

@@ -104,8 +104,10 @@ static void acc_add(u32 *acc, int n, const u32 *x, int nx, int sub) {
   }
 }
 
-/* sum_i x_i * 2 / d_i (mpf_ref.cauchy): x entry-major (K x L), d_i == 0 or x_i == 0 contribute nothing */
-int64_t oc_cauchy(const u32 *xm, const int64_t *xe, const int32_t *d, int K, int L, u32 *out) {
+/* h + sum_i x_i * 2 / d_i with one rounding (mpf_ref.cauchy): x entry-major (K x L); d_i == 0 or x_i == 0
+ * contribute nothing; hm == NULL or a zero h means no h */
+int64_t oc_cauchy(const u32 *xm, const int64_t *xe, const int32_t *d, int K, int L, const u32 *hm, int64_t he,
+                  u32 *out) {
   int64_t E = OC_EZ;
   int any = 0;
   for (int i = 0; i < K; ++i) {
@@ -113,9 +115,14 @@ int64_t oc_cauchy(const u32 *xm, const int64_t *xe, const int32_t *d, int K, int
     if (!any || xe[i] > E) E = xe[i];
     any = 1;
   }
+  const int hnz = hm && bitlen(hm, L) > 0;
   memset(out, 0, (size_t)L * 4);
-  if (!any) return OC_EZ;
-  u32 acc[OC_MAXL + 3], a[OC_MAXL + 1], q[OC_MAXL + 1], t[OC_MAXL + 1];
+  if (!any && !hnz) return OC_EZ;
+  int64_t G;
+  if (!any) G = he - 32;
+  else if (!hnz) G = E - 31;
+  else G = (E - 31 > he - 32) ? E - 31 : he - 32;
+  u32 acc[OC_MAXL + 3], a[OC_MAXL + 2], q[OC_MAXL + 1], t[OC_MAXL + 1];
   memset(acc, 0, sizeof(acc));
   for (int i = 0; i < K; ++i) {
     const u32 *x = xm + (size_t)i * L;
@@ -131,12 +138,23 @@ int64_t oc_cauchy(const u32 *xm, const int64_t *xe, const int32_t *d, int K, int
       q[k] = (u32)(cur / dd);
       rem = cur % dd;
     }
-    int64_t sh = E - xe[i];
+    int64_t sh = G + 31 - xe[i];
     if (sh >= 32 * (L + 1)) continue;
     shr(q, L + 1, (int)sh, t, L + 1);
     acc_add(acc, L + 3, t, L + 1, sx != (d[i] < 0));
   }
-  return oc_norm(acc, L + 3, E - 31, L, out);
+  if (hnz) {
+    memcpy(a, hm, (size_t)L * 4);
+    a[L] = 0;
+    int sh_ = is_neg(a, L);
+    if (sh_) negate(a, L);
+    int64_t tt = he - G; /* <= 32 */
+    if (tt >= 0) shl(a, L + 1, (int)tt, t, L + 1);
+    else if (-tt < 32 * (L + 1)) shr(a, L + 1, (int)(-tt), t, L + 1);
+    else memset(t, 0, (size_t)(L + 1) * 4);
+    acc_add(acc, L + 3, t, L + 1, sh_);
+  }
+  return oc_norm(acc, L + 3, G, L, out);
 }
 
 /* one pole-step entry from pre-aligned operands (mpf_ref.gemm_aligned): exact sum of K products of
@@ -218,7 +236,7 @@ static void *worker(void *arg) {
   while (t - t0 < j->seconds) {
     for (int r = 0; r < 64; ++r) {
       if (j->kind == 0) {
-        oc_cauchy(xm, xe, d, K, L, out);
+        oc_cauchy(xm, xe, d, K, L, bm, -(int64_t)(32 * L), out);
         units += K;
       } else {
         oc_dot_aligned(xm, -100, bm, -100, K, L, out);

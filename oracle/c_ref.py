@@ -18,7 +18,7 @@ def lib():
             B.build()
         lb = ctypes.CDLL(_PATH)
         P, I64, I = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int
-        lb.oc_cauchy.argtypes = [P, P, P, I, I, P]
+        lb.oc_cauchy.argtypes = [P, P, P, I, I, P, I64, P]
         lb.oc_cauchy.restype = I64
         lb.oc_dot_aligned.argtypes = [P, I64, P, I64, I, I, P]
         lb.oc_dot_aligned.restype = I64
@@ -51,13 +51,13 @@ def norm(W, n, e, L):
     return _result(out, e2, L)
 
 
-def cauchy(xs, ds, L):
+def cauchy(xs, ds, L, h=(0, R.EZ)):
     K = len(xs)
     xm = _limbs([m for m, _ in xs], L)
     xe = (ctypes.c_int64 * max(1, K))(*[e for _, e in xs])
     d = (ctypes.c_int32 * max(1, K))(*ds)
     out = (ctypes.c_uint32 * L)()
-    e = lib().oc_cauchy(xm, xe, d, K, L, out)
+    e = lib().oc_cauchy(xm, xe, d, K, L, _limbs([h[0]], L), h[1], out)
     return _result(out, e, L)
 
 

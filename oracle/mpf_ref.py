@@ -139,23 +139,35 @@ def mul(a, b, L):
     return norm(ma * mb, ea + eb, L)
 
 
-def cauchy(xs, ds, L):
-    """sum_i x_i * 2 / d_i for small nonzero integers d_i (d_i == 0: no term).
-    The x_i (all columns of the group, whatever d) are aligned to E = their largest exponent; the term i
-    is sign(x_i d_i) * floor(|x_i| * 2^(32 - sh_i) / |d_i|), sh_i = E - e_i, on the grid 2^(E - 31);
-    the terms are summed exactly and rounded once."""
+def cauchy(xs, ds, L, h=(0, EZ)):
+    """h + sum_i x_i * 2 / d_i for small nonzero integers d_i (d_i == 0: no term), one rounding.
+    E = the largest exponent of the nonzero x_i (all columns of the group, whatever d); the grid is
+    G = max(E - 31, e_h - 32) (E - 31 if h is zero, e_h - 32 if every x_i is zero); term i is
+    sign(x_i d_i) * floor(|x_i| * 2^(32 - sh_i) / |d_i|), sh_i = G + 31 - e_i >= 0, and h is truncated
+    toward zero onto 2^G (|m_h| 2^(e_h - G), e_h - G <= 32); the terms and h are summed exactly and
+    rounded once by norm."""
     nz = [e for m, e in xs if m != 0]
-    if not nz:
+    mh, eh = h
+    if not nz and mh == 0:
         return 0, EZ
-    E = max(nz)
+    if not nz:
+        G = eh - 32
+    elif mh == 0:
+        G = max(nz) - 31
+    else:
+        G = max(max(nz) - 31, eh - 32)
     acc = 0
     for (m, e), d in zip(xs, ds):
         if d == 0 or m == 0:
             continue
-        sh = E - e
+        sh = G + 31 - e
         q = (abs(m) << 32) // (abs(d) << sh) if sh <= 32 * L + 64 else 0
         acc += q if (m < 0) == (d < 0) else -q
-    return norm(acc, E - 31, L)
+    if mh:
+        t = eh - G
+        a = abs(mh) << t if t >= 0 else abs(mh) >> (-t)
+        acc += -a if mh < 0 else a
+    return norm(acc, G, L)
 
 
 def mom_flush(ys, nks, acc_prev, L):

@@ -3,8 +3,8 @@
 recursion()      -- restates reference blocksvibe/recursion_ref.py:11-44 (blocks_3d's fill_residues /
                     set_residue): any number type (Fraction for exact results).
 recursion_bits() -- the GPU kernels' bit-level semantics for the same loop, on top of oracle/mpf_ref.py:
-                    Cauchy step = mpf_ref.cauchy over all prime poles of the (src, j') block plus
-                    mpf_ref.add of h_inf; pole step = two rounded S x S products with the constant operand
+                    Cauchy step = mpf_ref.cauchy over all prime poles of the (src, j') block with
+                    h_inf in the same exact sum (one rounding); pole step = two rounded S x S products with the constant operand
                     pre-aligned per row (R, and q L with q L rounded once), the residue operand aligned
                     per column.  Structures are padded to Smax (zero entries: no effect on alignment).
 
@@ -80,8 +80,7 @@ def recursion_bits(h_inf, M, two_j12, two_j43, two_j_max, order, pseudo_parities
                 for b in range(Smax):
                     xs = [res.get((src, mm, tjp // 2, q, a, b), Z) for mm, q, d in terms]
                     ds = [d for mm, q, d in terms]
-                    c = R.cauchy(xs, ds, L)
-                    H[a][b] = R.add(c, h_inf.get((src, m, tjp // 2, a, b), Z), L)
+                    H[a][b] = R.cauchy(xs, ds, L, h_inf.get((src, m, tjp // 2, a, b), Z))
             # pole step: Y[b0][a] = sum_b R[b0][b] H[a][b];  out[a0][b0] = sum_a qL[a0][a] Y[b0][a]
             Ral = [R.align(Rp[b0]) for b0 in range(Smax)]
             qL = [[R.mul(Lp[a0][a], Qp, L) for a in range(Smax)] for a0 in range(Smax)]

@@ -1,0 +1,136 @@
+// CPU build of the gfx950 kernels' per-thread multiprecision helpers (blocksvibe_amd/csrc/mp.hpp), so that
+// tests can check them bit for bit against the oracle on inputs of any size without a GPU.  The device
+// intrinsics have host stand-ins (a ballot is the lane's own predicate, alignbit is a funnel shift).
+#include <math.h>
+#include <stdint.h>
+#define B3D_HOST_CHECK 1
+#define __device__
+#define __forceinline__ inline
+#include "../../blocksvibe_amd/csrc/mp.hpp"
+
+#define DISP(L, CALL)                                                                    \
+  switch (L) {                                                                           \
+    case 2: { constexpr int LL = 2; return CALL; }                                       \
+    case 4: { constexpr int LL = 4; return CALL; }                                       \
+    case 10: { constexpr int LL = 10; return CALL; }                                     \
+    case 34: { constexpr int LL = 34; return CALL; }                                     \
+    case 50: { constexpr int LL = 50; return CALL; }                                     \
+  }                                                                                      \
+  return -1;
+
+template <int L> static void copy_in(u32 (&d)[L], const uint32_t* s) { for (int k = 0; k < L; ++k) d[k] = s[k]; }
+template <int L> static void copy_out(uint32_t* d, const u32 (&s)[L]) { for (int k = 0; k < L; ++k) d[k] = s[k]; }
+
+template <int L> static int64_t add_t(const uint32_t* a, int64_t ea, const uint32_t* b, int64_t eb, uint32_t* r) {
+  u32 A[L], B[L], R[L];
+  copy_in(A, a);
+  copy_in(B, b);
+  const int64_t e = mp::add<L>(A, ea, B, eb, R);
+  copy_out(r, R);
+  return e;
+}
+
+template <int L> static int64_t mul_t(const uint32_t* a, int64_t ea, const uint32_t* b, int64_t eb, uint32_t* r) {
+  u32 A[L], B[L], R[L];
+  copy_in(A, a);
+  copy_in(B, b);
+  if (A[L - 1] == 0u || B[L - 1] == 0u) {
+    for (int k = 0; k < L; ++k) r[k] = 0;
+    return B3D_EZ;
+  }
+  mp::cneg(A, mp::to_mag(B));
+  u32 acc[2 * L + 2];
+  mp::zero(acc);
+  mp::mac_signed(acc, A, B);
+  const int64_t e = mp::norm<L, 2 * L + 2>(acc, ea + eb, R);
+  copy_out(r, R);
+  return e;
+}
+
+// norm of a two's complement integer given in LW limbs, sign-extended to the kernels' widths
+template <int L, int NW> static int64_t norm_w(const uint32_t* w, int LW, int64_t e, uint32_t* r) {
+  u32 acc[NW];
+  const u32 ext = (w[LW - 1] >> 31) ? 0xFFFFFFFFu : 0u;
+  for (int k = 0; k < NW; ++k) acc[k] = k < LW ? w[k] : ext;
+  u32 R[L];
+  const int64_t eo = mp::norm<L, NW>(acc, e, R);
+  copy_out(r, R);
+  return eo;
+}
+template <int L> static int64_t norm_t(const uint32_t* w, int LW, int64_t e, uint32_t* r) {
+  if (LW <= L + 3) return norm_w<L, L + 3>(w, LW, e, r);
+  if (LW <= 2 * L + 2) return norm_w<L, 2 * L + 2>(w, LW, e, r);
+  return norm_w<L, 2 * L + 4>(w, LW, e, r);
+}
+
+template <int L> static int64_t divs_t(const uint32_t* x, uint32_t d, uint32_t* q) {
+  u32 X[L], Q[L + 1];
+  copy_in(X, x);
+  mp::div_small<L>(X, d, Q);
+  for (int k = 0; k < L + 1; ++k) q[k] = Q[k];
+  return 0;
+}
+
+template <int L> static int64_t shr_t(const uint32_t* x, int s, int left, uint32_t* out) {
+  u32 X[L + 3];
+  for (int k = 0; k < L + 3; ++k) X[k] = x[k];
+  if (left) mp::shl(X, s);
+  else mp::shr(X, s);
+  for (int k = 0; k < L + 3; ++k) out[k] = X[k];
+  return 0;
+}
+
+template <int L>
+static int64_t cauchy_t(const int32_t* rm, const int64_t* re, int64_t rps, int64_t N, int64_t n, const int64_t* colg,
+                        const int32_t* coldg, int nc, int D, const int32_t* hm, const int64_t* he, int64_t hps,
+                        int64_t hidx, uint32_t* r) {
+  u32 R[L];
+  const int64_t e = mp::cauchy_entry<L>(rm, re, rps, N, n, colg, coldg, nc, D, hm, he, hps, hidx, R);
+  copy_out(r, R);
+  return e;
+}
+
+template <int L>
+static int64_t dot_t(const int32_t* am, int64_t aps, int64_t abase, int64_t ea, const int32_t* xm, const int64_t* xe,
+                     int64_t xps, int64_t xbase, int64_t xstride, int S, uint32_t* r) {
+  u32 R[L];
+  const int64_t e = mp::small_dot<L>(am, aps, abase, ea, xm, xe, xps, xbase, xstride, S, R);
+  copy_out(r, R);
+  return e;
+}
+
+template <int L>
+static int64_t mom_t(const int32_t* sm, const int64_t* se, int64_t sps, int W, int64_t nrows, int64_t N, int64_t row,
+                     int64_t c, int n0, int k, int R1, const uint32_t* NK, int64_t nkps, int KL, const int32_t* NKb,
+                     uint32_t* r) {
+  u32 R[L];
+  const int64_t e = mp::mom_window<L, 4>(sm, se, sps, W, nrows, N, row, c, n0, k, R1, NK, nkps, KL, NKb, R);
+  copy_out(r, R);
+  return e;
+}
+
+extern "C" {
+int64_t hc_add(int L, const uint32_t* a, int64_t ea, const uint32_t* b, int64_t eb, uint32_t* r) {
+  DISP(L, add_t<LL>(a, ea, b, eb, r))
+}
+int64_t hc_mul(int L, const uint32_t* a, int64_t ea, const uint32_t* b, int64_t eb, uint32_t* r) {
+  DISP(L, mul_t<LL>(a, ea, b, eb, r))
+}
+int64_t hc_norm(int L, const uint32_t* w, int LW, int64_t e, uint32_t* r) { DISP(L, norm_t<LL>(w, LW, e, r)) }
+int64_t hc_div(int L, const uint32_t* x, uint32_t d, uint32_t* q) { DISP(L, divs_t<LL>(x, d, q)) }
+int64_t hc_shift(int L, const uint32_t* x, int s, int left, uint32_t* out) { DISP(L, shr_t<LL>(x, s, left, out)) }
+int64_t hc_cauchy(int L, const int32_t* rm, const int64_t* re, int64_t rps, int64_t N, int64_t n, const int64_t* colg,
+                  const int32_t* coldg, int nc, int D, const int32_t* hm, const int64_t* he, int64_t hps, int64_t hidx,
+                  uint32_t* r) {
+  DISP(L, cauchy_t<LL>(rm, re, rps, N, n, colg, coldg, nc, D, hm, he, hps, hidx, r))
+}
+int64_t hc_dot(int L, const int32_t* am, int64_t aps, int64_t abase, int64_t ea, const int32_t* xm, const int64_t* xe,
+               int64_t xps, int64_t xbase, int64_t xstride, int S, uint32_t* r) {
+  DISP(L, dot_t<LL>(am, aps, abase, ea, xm, xe, xps, xbase, xstride, S, r))
+}
+int64_t hc_mom(int L, const int32_t* sm, const int64_t* se, int64_t sps, int W, int64_t nrows, int64_t N, int64_t row,
+               int64_t c, int n0, int k, int R1, const uint32_t* NK, int64_t nkps, int KL, const int32_t* NKb,
+               uint32_t* r) {
+  DISP(L, mom_t<LL>(sm, se, sps, W, nrows, N, row, c, n0, k, R1, NK, nkps, KL, NKb, r))
+}
+}

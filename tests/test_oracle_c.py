@@ -52,7 +52,12 @@ def test_cauchy(L):
         ds = [rng.choice([0, 1, -1, 2, (1 << 15) - 1, -((1 << 15) - 1)] + [rng.randint(-300, 300)])
               for _ in range(K)]
         assert C.cauchy(xs, ds, L) == R.cauchy(xs, ds, L), (trial, K, spread)
+        h = rnum(rng, L, rng.choice([0, 5, 40, 32 * L + 100]), zero=0.1)
+        h = (h[0], h[1] + rng.choice([0, 0, -60, 60]))
+        assert C.cauchy(xs, ds, L, h) == R.cauchy(xs, ds, L, h), (trial, K, spread, "h")
     assert C.cauchy([(0, R.EZ)] * 3, [1, 2, 3], L) == (0, R.EZ)
+    h = rnum(rng, L)
+    assert C.cauchy([(0, R.EZ)] * 3, [1, 2, 3], L, h) == h == R.cauchy([(0, R.EZ)] * 3, [1, 2, 3], L, h)
 
 
 @pytest.mark.parametrize("L", [2, 4, 10, 34])

@@ -87,6 +87,11 @@ def test_gemm_and_cauchy_bounds():
         exact = sum(R.value(*x) * F(2, d) for x, d in zip(xs, ds) if d)
         _, E = R.align(xs)
         assert abs(R.value(m, e) - exact) <= 3 * K * F(2) ** E + (F(2) ** e if m else 0)
+        # with h in the same sum: every term and h truncated on the grid G, one rounding
+        h = rp(rng, L, 60)
+        m, e = R.cauchy(xs, ds, L, h)
+        G = max(E - 31, h[1] - 32)
+        assert abs(R.value(m, e) - (exact + R.value(*h))) <= (K + 1) * F(2) ** G + (F(2) ** e / 2 if m else 0)
 
 
 def test_mom_flush_exact_when_no_truncation():
