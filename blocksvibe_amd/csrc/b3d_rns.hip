@@ -26,15 +26,6 @@ static inline unsigned grid_for(int64_t n, int th) {
   return (unsigned)b;
 }
 
-// x mod p for 0 <= x < 2^53, through a double reciprocal (one correction either way)
-__device__ __forceinline__ uint32_t mod_small(uint64_t x, uint32_t p, double rp) {
-  const double q = floor((double)x * rp);
-  int64_t r = (int64_t)x - (int64_t)q * (int64_t)p;
-  if (r < 0) r += p;
-  if (r >= (int64_t)p) r -= p;
-  return (uint32_t)r;
-}
-
 // signed v (|v| < 2^52) mod p in [0, p)
 __device__ __forceinline__ uint32_t mod_signed(int64_t v, uint32_t p, double rp) {
   const double q = floor((double)v * rp);
@@ -63,49 +54,113 @@ __global__ __launch_bounds__(256) void k_line_max(const int32_t* __restrict__ m,
   }
 }
 
-// (2) residue digits.  Output planes out[(t*2 + d) * plane + (b * Lp + line) * Kp + k], d = 0 high digit,
-// 1 low digit, zero in the padding (line >= lines or k >= K).
+// (2) residue digits on MFMA.  x mod p_t = sum_j byte_j(x) c_tj mod p_t with c_tj = 2^(8 j) mod p_t
+// (balanced, split into int8 digits c = 256 ch + cl): an int8 GEMM of the entries' bytes (stored as
+// byte - 128 = byte ^ 0x80, signed) against the constant weights w8 = [ch | cl][prime][KB] (KB = 4L bytes
+// padded to a multiple of 64), corrected by wcorr_t = 128 sum_j c_tj.  One wave converts 64 consecutive
+// entries t (t enumerates (b, line, k), k fastest, zero in the padding line >= lines or k >= K):
+//   phase 1  lane = entry: load, align to the line exponent, magnitude bytes (^ 0x80) into LDS
+//   phase 2  per 16 primes: 4 entry tiles x KB/64 steps x 2 digit halves of v_mfma_i32_16x16x64_i8
+//   phase 3  Y = 256 Yh + Yl + wcorr mod p, sign, balanced residue -> int8 digits (h, l), staged in LDS
+//            and written as 64 contiguous bytes per output plane.
+// Output planes out[(t*2 + d) * plane + entry], d = 0 high digit, 1 low digit.
 template <int L>
-__global__ __launch_bounds__(256) void k_to_residues(const int32_t* __restrict__ m, const i64* __restrict__ e, i64 ps,
-                                                     i64 nb, i64 lines, i64 K, i64 sb, i64 sl, i64 sk,
-                                                     const i64* __restrict__ Eline, i64 estride, int Lp, int Kp, int nmod,
-                                                     const uint16_t* __restrict__ primes,
-                                                     const uint32_t* __restrict__ pw, const double* __restrict__ rp,
-                                                     int8_t* __restrict__ out, i64 plane) {
-  const i64 total = nb * (i64)Lp * Kp;
-  for (i64 t = blockIdx.x * (i64)blockDim.x + threadIdx.x; t < total; t += (i64)gridDim.x * blockDim.x) {
-    const i64 k = t % Kp, line = (t / Kp) % Lp, b = t / ((i64)Kp * Lp);
-    u32 x[L];
-    bool ng = false;
-    bool nz = false;
-    if (line < lines && k < K) {
-      const i64 idx = b * sb + line * sl + k * sk;
-      mp::load(x, m, ps, idx);
-      if (x[L - 1] != 0u) {
-        nz = true;
-        ng = mp::to_mag(x) != 0u;
-        const i64 sh = Eline[b * estride + line] - e[idx];
-        mp::shr(x, sh > 32 * L ? 32 * L : (int)sh);
+__global__ __launch_bounds__(64) void k_to_residues(const int32_t* __restrict__ m, const i64* __restrict__ e, i64 ps,
+                                                    i64 nb, i64 lines, i64 K, i64 sb, i64 sl, i64 sk,
+                                                    const i64* __restrict__ Eline, i64 estride, int Lp, int Kp,
+                                                    int nmod, const uint16_t* __restrict__ primes,
+                                                    const int8_t* __restrict__ w8, const int32_t* __restrict__ wcorr,
+                                                    const double* __restrict__ rp, int8_t* __restrict__ out,
+                                                    i64 plane) {
+  constexpr int KB = (4 * L + 63) / 64 * 64;  // bytes per entry, padded
+  constexpr int NKS = KB / 64;                // MFMA K steps
+  constexpr int ROW = KB + 16;                // LDS row stride (bytes)
+  const int np16 = (nmod + 15) / 16 * 16;
+  __shared__ __align__(16) int8_t xb[64 * ROW];
+  __shared__ __align__(16) int8_t stage[32 * 64];
+  __shared__ u32 sgn[64];
+  const int lane = threadIdx.x;
+  const int r = lane & 15, g = lane >> 4;
+  const i64 total = nb * (i64)Lp * Kp;  // a multiple of 64 (Kp is)
+  for (i64 t0 = (i64)blockIdx.x * 64; t0 < total; t0 += (i64)gridDim.x * 64) {
+    // phase 1
+    {
+      const i64 t = t0 + lane;
+      const i64 k = t % Kp, line = (t / Kp) % Lp, b = t / ((i64)Kp * Lp);
+      u32 x[L];
+      mp::zero(x);
+      u32 sm = 0;
+      if (line < lines && k < K) {
+        const i64 idx = b * sb + line * sl + k * sk;
+        mp::load(x, m, ps, idx);
+        if (x[L - 1] != 0u) {
+          sm = mp::to_mag(x);
+          const i64 sh = Eline[b * estride + line] - e[idx];
+          mp::shr(x, sh > 32 * L ? 32 * L : (int)sh);
+        }
       }
-    }
-    for (int tm = 0; tm < nmod; ++tm) {
-      int h = 0, l = 0;
-      if (nz) {
-        const uint32_t p = primes[tm];
-        const uint32_t* pwt = pw + (i64)tm * L;
-        // sum_q x_q (2^(32 q) mod p): L products < 2^47, exact in FP64 (full rate) for L <= 64
-        double s = 0.0;
+      u32* row = (u32*)(xb + lane * ROW);
 #pragma unroll
-        for (int q = 0; q < L; ++q) s = fma((double)x[q], (double)pwt[q], s);
-        uint32_t r = mod_small((uint64_t)s, p, rp[tm]);
-        if (ng && r) r = p - r;
-        int rb = (int)r;
-        if (rb > (int)(p >> 1)) rb -= (int)p;  // balanced, |rb| < 2^14
-        h = (rb + 128) >> 8;
-        l = rb - 256 * h;
+      for (int q = 0; q < KB / 4; ++q) row[q] = q < L ? (x[q] ^ 0x80808080u) : 0u;
+      sgn[lane] = sm;
+    }
+    __syncthreads();
+    v4i A[4][NKS];
+#pragma unroll
+    for (int et = 0; et < 4; ++et)
+#pragma unroll
+      for (int ks = 0; ks < NKS; ++ks) A[et][ks] = *(const v4i*)(xb + (et * 16 + r) * ROW + ks * 64 + g * 16);
+    for (int pt = 0; pt < np16; pt += 16) {
+      // phase 2
+      v4i Bh[NKS], Bl[NKS];
+#pragma unroll
+      for (int ks = 0; ks < NKS; ++ks) {
+        Bh[ks] = *(const v4i*)(w8 + ((i64)0 * np16 + pt + r) * KB + ks * 64 + g * 16);
+        Bl[ks] = *(const v4i*)(w8 + ((i64)1 * np16 + pt + r) * KB + ks * 64 + g * 16);
       }
-      out[(i64)(tm * 2 + 0) * plane + t] = (int8_t)h;
-      out[(i64)(tm * 2 + 1) * plane + t] = (int8_t)l;
+      const int tm = pt + r;
+      const bool live = tm < nmod;
+      const uint32_t p = live ? primes[tm] : 1u;
+      const double rpt = live ? rp[tm] : 1.0;
+      const i64 corr = live ? wcorr[tm] : 0;
+#pragma unroll
+      for (int et = 0; et < 4; ++et) {
+        v4i ah = {0, 0, 0, 0}, al = {0, 0, 0, 0};
+#pragma unroll
+        for (int ks = 0; ks < NKS; ++ks) {
+          ah = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[et][ks], Bh[ks], ah, 0, 0, 0);
+          al = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[et][ks], Bl[ks], al, 0, 0, 0);
+        }
+        // phase 3: entries et*16 + 4g + i, prime tm
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int ent = et * 16 + 4 * g + i;
+          const double y = (double)((i64)ah[i] * 256 + (i64)al[i] + corr);   // |y| < 2^31: exact
+          const double q = floor(y * rpt);
+          double rr = fma(-q, (double)p, y);
+          rr = rr < 0.0 ? rr + (double)p : rr;
+          rr = rr >= (double)p ? rr - (double)p : rr;
+          int rb = (int)rr;
+          if (sgn[ent] && rb) rb = (int)p - rb;
+          if (rb > (int)(p >> 1)) rb -= (int)p;  // balanced, |rb| < 2^14
+          const int h = (rb + 128) >> 8;
+          const int l = rb - 256 * h;
+          stage[(r * 2 + 0) * 64 + ent] = (int8_t)h;
+          stage[(r * 2 + 1) * 64 + ent] = (int8_t)l;
+        }
+      }
+      __syncthreads();
+      {
+        const int pd = lane >> 1, half = lane & 1;   // plane (prime pt + pd/2, digit pd%2), 32 bytes
+        const int tq = pt + (pd >> 1);
+        if (tq < nmod) {
+          const v4i* src = (const v4i*)(stage + pd * 64 + half * 32);
+          v4i* dst = (v4i*)(out + (i64)(tq * 2 + (pd & 1)) * plane + t0 + half * 32);
+          dst[0] = src[0];
+          dst[1] = src[1];
+        }
+      }
+      __syncthreads();
     }
   }
 }
@@ -155,60 +210,109 @@ __global__ __launch_bounds__(256) void k_rns_mma(const int8_t* __restrict__ A8, 
   }
 }
 
-// (4) CRT lift and normalisation; output entry (b, i, j) at b*csb + i*csi + j*csj (+ column offset j0)
+// (4) CRT lift on MFMA and normalisation.  With y_t = c_t (M/p_t)^-1 mod p_t taken balanced (|y_t| < 2^14,
+// split y_t = 256 h_t + l_t into int8 digits) and q = round(sum_t y_t / p_t) (|q| <= nmod / 2 < 128),
+//   C = sum_t y_t M/p_t - q M
+// is one int8 GEMM against the constant signed radix-256 digits of M/p_t (row l_t), of 256 M/p_t (row h_t)
+// and of M (row -q): c8[byte column][k], k < CK.  Byte columns run along the MFMA rows, so each lane holds
+// the sums of four consecutive byte columns = one 32-bit limb's worth (|sum| < 2^48 in int64).  One wave
+// lifts 16 outputs per pass:
+//   phase 1  64 lanes: residues -> balanced y digits (output r = lane % 16, primes t = lane / 16 mod 4)
+//            into LDS, q from the per-lane partial sums of y_t / p_t
+//   phase 2  per 16 byte columns: CK/64 steps of v_mfma_i32_16x16x64_i8 -> limb partial sums into LDS
+//   phase 3  lanes 0..15: carry propagation into 2L + 2 limbs, one rounding, store.
+// Output entry (b, i, j) at b*csb + i*csi + j*csj (+ column offset j0).
 template <int L>
-__global__ __launch_bounds__(256) void k_crt(const uint16_t* __restrict__ Cr, i64 nb, int M, int N, int nmod,
-                                             const uint16_t* __restrict__ primes, const uint16_t* __restrict__ inv,
-                                             const double* __restrict__ rp, const uint32_t* __restrict__ Mp,
-                                             const uint32_t* __restrict__ Mfull, const i64* __restrict__ EA,
-                                             int a_shared, const i64* __restrict__ EB, i64 j0, i64 Nfull,
-                                             int32_t* __restrict__ cm, i64* __restrict__ ce, i64 cps, i64 csb,
-                                             i64 csi, i64 csj) {
+constexpr int crt_ckmax() {
+  return (2 * ((2 * (32 * L - 2) + 20) / 14 + 2) + 1 + 63) / 64 * 64;
+}
+
+template <int L>
+__global__ __launch_bounds__(64) void k_crt(const uint16_t* __restrict__ Cr, i64 nb, int M, int N, int nmod,
+                                            const uint16_t* __restrict__ primes, const uint16_t* __restrict__ inv,
+                                            const double* __restrict__ rp, const int8_t* __restrict__ c8, int CK,
+                                            const i64* __restrict__ EA, int a_shared, const i64* __restrict__ EB,
+                                            i64 j0, i64 Nfull, int32_t* __restrict__ cm, i64* __restrict__ ce,
+                                            i64 cps, i64 csb, i64 csi, i64 csj) {
   constexpr int LW = 2 * L + 2;
+  constexpr int CN = (4 * LW + 2 + 15) / 16 * 16;
+  constexpr int NRT = CN / 16;
+  constexpr int VW = 4 * NRT + 1;
+  constexpr int CKMAX = crt_ckmax<L>();
+  constexpr int YROW = CKMAX + 16;
+  __shared__ __align__(16) int8_t yv[16 * YROW];
+  __shared__ i64 V[16 * VW];
+  __shared__ double qpart[64];
+  const int lane = threadIdx.x, r = lane & 15, g = lane >> 4;
   const i64 total = nb * (i64)M * N;
-  for (i64 t = blockIdx.x * (i64)blockDim.x + threadIdx.x; t < total; t += (i64)gridDim.x * blockDim.x) {
-    const i64 j = t % N, i = (t / N) % M, b = t / ((i64)N * M);
-    const i64 ea = EA[(a_shared ? 0 : b) * M + i];
-    const i64 eb = EB[b * Nfull + j0 + j];
-    u32 out[L];
-    i64 eo = B3D_EZ;
-    if (ea == B3D_EZ || eb == B3D_EZ) {
-      mp::zero(out);
-    } else {
-      u32 acc[LW];
-      mp::zero(acc);
-      double qs = 0.0;
-      for (int tm = 0; tm < nmod; ++tm) {
-        const uint32_t p = primes[tm];
-        const uint32_t c = Cr[((i64)tm * nb + b) * M * N + i * (i64)N + j];
-        const uint32_t y = mod_small((uint64_t)c * inv[tm], p, rp[tm]);
-        qs += (double)y * rp[tm];
-        const uint32_t* mp_t = Mp + (i64)tm * LW;
-        u64 cy = 0;
-#pragma unroll
-        for (int q = 0; q < LW; ++q) {
-          const u64 v = (u64)y * mp_t[q] + acc[q] + cy;
-          acc[q] = (u32)v;
-          cy = v >> 32;
-        }
+  const int nks = CK / 64;
+  for (i64 o0 = (i64)blockIdx.x * 16; o0 < total; o0 += (i64)gridDim.x * 16) {
+    const i64 o = o0 + r;
+    const bool live = o < total;
+    // phase 1
+    double qs = 0.0;
+    int8_t* yrow = yv + r * YROW;
+    for (int t = g; t < nmod; t += 4) {
+      int yb = 0;
+      if (live) {
+        const uint32_t p = primes[t];
+        const double pd = (double)p;
+        const double v = (double)((uint32_t)Cr[(i64)t * total + o] * (uint32_t)inv[t]);   // < 2^30
+        double y = fma(-floor(v * rp[t]), pd, v);
+        y = y < 0.0 ? y + pd : y;
+        y = y >= pd ? y - pd : y;
+        yb = (int)y;
+        if (yb > (int)(p >> 1)) yb -= (int)p;
+        qs += (double)yb * rp[t];
       }
-      const u32 qq = (u32)rint(qs);
-      // acc -= qq * M
-      u64 cy = 0;
-      u32 bor = 0;
-#pragma unroll
-      for (int q = 0; q < LW; ++q) {
-        const u64 prod = (u64)qq * Mfull[q] + cy;
-        cy = prod >> 32;
-        const u64 v = (u64)acc[q] - (u64)(u32)prod - (u64)bor;
-        acc[q] = (u32)v;
-        bor = (u32)(v >> 63);
-      }
-      eo = mp::norm<L, LW>(acc, ea + eb, out);
+      const int h = (yb + 128) >> 8;
+      yrow[2 * t] = (int8_t)h;
+      yrow[2 * t + 1] = (int8_t)(yb - 256 * h);
     }
-    const i64 o = b * csb + i * csi + (j0 + j) * csj;
-    mp::store(out, cm, cps, o);
-    ce[o] = eo;
+    for (int k = 2 * nmod + 1 + g; k < CK; k += 4) yrow[k] = 0;
+    qpart[lane] = qs;
+    __syncthreads();
+    if (g == 0) {
+      const double sq = qpart[r] + qpart[r + 16] + qpart[r + 32] + qpart[r + 48];
+      yrow[2 * nmod] = (int8_t)(-(int)rint(sq));
+    }
+    __syncthreads();
+    // phase 2
+    for (int rt = 0; rt < NRT; ++rt) {
+      v4i acc = {0, 0, 0, 0};
+      for (int ks = 0; ks < nks; ++ks) {
+        const v4i a = *(const v4i*)(c8 + (i64)(rt * 16 + r) * CK + ks * 64 + g * 16);
+        const v4i b = *(const v4i*)(yrow + ks * 64 + g * 16);
+        acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, acc, 0, 0, 0);
+      }
+      V[r * VW + 4 * rt + g] = (i64)acc[0] + (i64)acc[1] * 256 + (i64)acc[2] * 65536 + (i64)acc[3] * 16777216;
+    }
+    __syncthreads();
+    // phase 3
+    if (g == 0 && live) {
+      const i64 j = o % N, i = (o / N) % M, b = o / ((i64)N * M);
+      const i64 ea = EA[(a_shared ? 0 : b) * M + i];
+      const i64 eb = EB[b * Nfull + j0 + j];
+      u32 out[L];
+      i64 eo = B3D_EZ;
+      if (ea == B3D_EZ || eb == B3D_EZ) {
+        mp::zero(out);
+      } else {
+        u32 acc[LW];
+        i64 carry = 0;
+#pragma unroll
+        for (int k = 0; k < LW; ++k) {
+          const i64 v = V[r * VW + k] + carry;
+          acc[k] = (u32)v;
+          carry = v >> 32;   // arithmetic: floor division by 2^32
+        }
+        eo = mp::norm<L, LW>(acc, ea + eb, out);
+      }
+      const i64 oo = b * csb + i * csi + (j0 + j) * csj;
+      mp::store(out, cm, cps, oo);
+      ce[oo] = eo;
+    }
+    __syncthreads();
   }
 }
 
@@ -224,8 +328,8 @@ int64_t b3d_rns_workspace_bytes(int nmod, int64_t nbA, int64_t nb, int64_t M, in
   return bytes + 1024;
 }
 
-int b3d_rns_gemm(int L, int nmod, const uint16_t* primes, const uint32_t* pw, const uint16_t* inv, const double* rp,
-                 const uint32_t* Mp, const uint32_t* Mfull, int64_t nb, int64_t M, int64_t N, int64_t K, int a_shared,
+int b3d_rns_gemm(int L, int nmod, const uint16_t* primes, const int8_t* w8, const int32_t* wcorr, const uint16_t* inv,
+                 const double* rp, const int8_t* c8, int CK, int64_t nb, int64_t M, int64_t N, int64_t K, int a_shared,
                  const int32_t* Am, const int64_t* Ae, int64_t aps, int64_t asb, int64_t asi, int64_t ask,
                  const int32_t* Bm, const int64_t* Be, int64_t bps, int64_t bsb, int64_t bsk, int64_t bsj, int32_t* Cm,
                  int64_t* Ce, int64_t cps, int64_t csb, int64_t csi, int64_t csj, int64_t Nc, int64_t* EB,
@@ -237,6 +341,11 @@ int b3d_rns_gemm(int L, int nmod, const uint16_t* primes, const uint32_t* pw, co
   }
   if (nb > 65535) {
     b3d::set_error("b3d_rns_gemm: batch must be < 65536");
+    return 2;
+  }
+  const int ckmax = (2 * ((2 * (32 * L - 2) + 20) / 14 + 2) + 1 + 63) / 64 * 64;   // crt_ckmax<L>()
+  if (nmod <= 0 || nmod > 255 || CK % 64 != 0 || CK < 2 * nmod + 1 || CK > ckmax) {
+    b3d::set_error("b3d_rns_gemm: CRT digit table does not fit (need 2 nmod + 1 <= CK <= bound, CK % 64 == 0)");
     return 2;
   }
   if (Nc <= 0 || Nc > N) Nc = N;
@@ -259,8 +368,9 @@ int b3d_rns_gemm(int L, int nmod, const uint16_t* primes, const uint32_t* pw, co
   const int64_t aplane = nbA * Mpad * Kpad, bplane = nb * (int64_t)Npad * Kpad;
   // A: rows
   k_line_max<<<grid_for(nbA * M, 256), 256, 0, s>>>(Am, Ae, (int64_t)(L - 1) * aps, nbA, M, K, asb, asi, ask, EA);
-  B3D_DISPATCH_L(L, k_to_residues<LL><<<grid_for(nbA * (int64_t)Mpad * Kpad, 256), 256, 0, s>>>(
-                        Am, Ae, aps, nbA, M, K, asb, asi, ask, EA, M, Mpad, Kpad, nmod, primes, pw, rp, A8, aplane));
+  B3D_DISPATCH_L(L, k_to_residues<LL><<<grid_for(nbA * (int64_t)Mpad * Kpad, 64), 64, 0, s>>>(
+                        Am, Ae, aps, nbA, M, K, asb, asi, ask, EA, M, Mpad, Kpad, nmod, primes, w8, wcorr, rp, A8,
+                        aplane));
   // B: columns (all of them once: exponents over k per column)
   k_line_max<<<grid_for(nb * N, 256), 256, 0, s>>>(Bm, Be, (int64_t)(L - 1) * bps, nb, N, K, bsb, bsj, bsk, EB);
   if (b3d::check_launch("b3d_rns_gemm (prep)")) return 1;
@@ -268,14 +378,14 @@ int b3d_rns_gemm(int L, int nmod, const uint16_t* primes, const uint32_t* pw, co
     const int64_t nc = (N - j0) < Nc ? (N - j0) : Nc;
     const int Np = (int)((nc + 63) / 64 * 64);
     // column chunk [j0, j0 + nc): shift the B base pointer, column exponents indexed with the full N
-    B3D_DISPATCH_L(L, k_to_residues<LL><<<grid_for(nb * (int64_t)Np * Kpad, 256), 256, 0, s>>>(
+    B3D_DISPATCH_L(L, k_to_residues<LL><<<grid_for(nb * (int64_t)Np * Kpad, 64), 64, 0, s>>>(
                           Bm + j0 * bsj, Be + j0 * bsj, bps, nb, nc, K, bsb, bsj, bsk, EB + j0, N, Np, Kpad, nmod,
-                          primes, pw, rp, B8, nb * (int64_t)Np * Kpad));
+                          primes, w8, wcorr, rp, B8, nb * (int64_t)Np * Kpad));
     dim3 grid((unsigned)(((Mpad + 15) / 16) * ((Np + 63) / 64)), (unsigned)nmod, (unsigned)nb);
     k_rns_mma<<<grid, 256, 0, s>>>(A8, aplane, a_shared, B8, nb * (int64_t)Np * Kpad, Mpad, Np, Kpad, (int)M,
                                    (int)nc, primes, rp, Cr, nb);
-    B3D_DISPATCH_L(L, k_crt<LL><<<grid_for(nb * M * nc, 256), 256, 0, s>>>(
-                          Cr, nb, (int)M, (int)nc, nmod, primes, inv, rp, Mp, Mfull, EA, a_shared, EB, j0, N, Cm, Ce,
+    B3D_DISPATCH_L(L, k_crt<LL><<<grid_for(nb * M * nc, 16), 64, 0, s>>>(
+                          Cr, nb, (int)M, (int)nc, nmod, primes, inv, rp, c8, CK, EA, a_shared, EB, j0, N, Cm, Ce,
                           cps, csb, csi, csj));
     if (b3d::check_launch("b3d_rns_gemm (chunk)")) return 1;
   }

@@ -5,8 +5,9 @@ blocksvibe/smallgemm.py:1-6 "A aligned per row, B per column, exact products, on
 C[i, j] = round(sum_k Abar[i, k] Bbar[k, j]) with Abar the row-aligned mantissas of A (exponent eA[i]) and
 Bbar the column-aligned mantissas of B (exponent eB[j]).
 
-Tables: the nmod largest primes below 2^15 with product M > 2^(2 NB + ceil(log2 K) + 4); residues of the
-limb weights 2^(32 l) mod p; the CRT constants (M/p)^-1 mod p and the limbs of M/p and M.
+Tables: the nmod largest primes below 2^15 with product M > 2^(2 NB + ceil(log2 K) + 4); int8 weights of
+the byte values 2^(8 j) mod p for the MFMA residue conversion; the CRT constants (M/p)^-1 mod p and the
+signed radix-256 digits of M/p and M for the MFMA CRT lift.
 """
 import math
 
@@ -45,6 +46,60 @@ def n_moduli(L, K):
     raise ValueError("not enough primes")
 
 
+def conversion_weights(ps, L):
+    """int8 weights of the MFMA residue conversion: for byte j < 4L of a magnitude and prime p_t,
+    c = 2^(8 j) mod p_t taken balanced (|c| <= p_t / 2 < 2^14) and split c = 256 ch + cl (cl in [-128, 127],
+    |ch| <= 64): w8[0, t, j] = ch, w8[1, t, j] = cl, bytes padded to a multiple of 64 and primes to a
+    multiple of 16 with zeros.  The kernel feeds byte - 128; wcorr[t] = 128 sum_j c restores sum_j byte_j c."""
+    KB = (4 * L + 63) // 64 * 64
+    n16 = (len(ps) + 15) // 16 * 16
+    w8 = np.zeros((2, n16, KB), dtype=np.int8)
+    wcorr = np.zeros(n16, dtype=np.int32)
+    for t, p in enumerate(ps):
+        c = np.array([pow(2, 8 * j, p) for j in range(4 * L)], dtype=np.int64)
+        c = np.where(c > p // 2, c - p, c)
+        ch = (c + 128) >> 8
+        cl = c - 256 * ch
+        w8[0, t, :4 * L] = ch
+        w8[1, t, :4 * L] = cl
+        wcorr[t] = 128 * int(c.sum())
+    return w8, wcorr
+
+
+def signed_digits(X, n):
+    """X >= 0 as n signed radix-256 digits in [-128, 127] (X = sum d_j 256^j)"""
+    d = []
+    for _ in range(n):
+        v = X & 255
+        if v >= 128:
+            v -= 256
+        d.append(v)
+        X = (X - v) >> 8
+    assert X == 0, "not enough digits"
+    return d
+
+
+def crt_digits(ps, L):
+    """int8 matrix of the MFMA CRT lift, [byte column][k]: k = 2t + 1 (low digit l_t of y_t): the signed
+    radix-256 digits of M/p_t; k = 2t (high digit h_t, weight 256): the same digits one column up;
+    k = 2 nmod (the coefficient -q): the digits of M.  Byte columns cover 2L + 2 limbs plus two, rounded up
+    to 16; k is padded with zeros to a multiple of 64."""
+    n = len(ps)
+    LW = 2 * L + 2
+    CN = (4 * LW + 2 + 15) // 16 * 16
+    CK = (2 * n + 1 + 63) // 64 * 64
+    M = 1
+    for p in ps:
+        M *= p
+    c8 = np.zeros((CN, CK), dtype=np.int8)
+    for t, p in enumerate(ps):
+        d = signed_digits(M // p, CN - 1)
+        c8[:CN - 1, 2 * t + 1] = d
+        c8[1:, 2 * t] = d
+    c8[:, 2 * n] = signed_digits(M, CN)
+    return c8
+
+
 def tables(L, nmod):
     key = (L, nmod)
     if key in _TABLES:
@@ -55,20 +110,18 @@ def tables(L, nmod):
         M *= p
     LW = 2 * L + 2
     assert M.bit_length() < 32 * LW - 1, "residue product does not fit the CRT accumulator"
-    pw = np.array([[pow(2, 32 * l, p) for l in range(L)] for p in ps], dtype=np.uint32)
+    w8, wcorr = conversion_weights(ps, L)
     inv = np.array([pow((M // p) % p, -1, p) for p in ps], dtype=np.uint16)
     rp = np.array([1.0 / p for p in ps], dtype=np.float64)
-    mask = (1 << 32) - 1
-    Mp = np.array([[((M // p) >> (32 * q)) & mask for q in range(LW)] for p in ps], dtype=np.uint32)
-    Mf = np.array([(M >> (32 * q)) & mask for q in range(LW)], dtype=np.uint32)
+    c8 = crt_digits(ps, L)
     d = mpf.dev()
     t = {"nmod": nmod,
          "primes": torch.from_numpy(np.array(ps, dtype=np.uint16).view(np.int16)).to(d),
-         "pw": torch.from_numpy(pw.view(np.int32)).to(d),
+         "w8": torch.from_numpy(w8).to(d),
+         "wcorr": torch.from_numpy(wcorr).to(d),
          "inv": torch.from_numpy(inv.view(np.int16)).to(d),
          "rp": torch.from_numpy(rp).to(d),
-         "Mp": torch.from_numpy(Mp.view(np.int32)).to(d),
-         "Mf": torch.from_numpy(Mf.view(np.int32)).to(d)}
+         "c8": torch.from_numpy(c8).to(d), "CK": int(c8.shape[1])}
     _TABLES[key] = t
     return t
 
@@ -94,7 +147,8 @@ def gemm_strided(L, nb, M, N, K, A, a_strides, a_shared, B, b_strides, C, c_stri
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=A.e.device)
     EB = torch.empty(nb * N, dtype=torch.int64, device=A.e.device)
     P = lambda x: x.data_ptr()
-    _lib.call("b3d_rns_gemm", L, nmod, P(t["primes"]), P(t["pw"]), P(t["inv"]), P(t["rp"]), P(t["Mp"]), P(t["Mf"]),
+    _lib.call("b3d_rns_gemm", L, nmod, P(t["primes"]), P(t["w8"]), P(t["wcorr"]), P(t["inv"]), P(t["rp"]),
+              P(t["c8"]), t["CK"],
               nb, M, N, K, 1 if a_shared else 0,
               P(A.m), P(A.e), A.m[0].numel(), *a_strides,
               P(B.m), P(B.e), B.m[0].numel(), *b_strides,

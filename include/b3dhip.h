@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-#define B3D_ABI_VERSION 1
+#define B3D_ABI_VERSION 3
 
 /* Last error message of the calling thread (empty if none). */
 const char* b3d_last_error(void);
@@ -93,12 +93,16 @@ int64_t b3d_rns_workspace_bytes(int nmod, int64_t nbA, int64_t nb, int64_t M, in
 
 /* C[b] = A[b] B[b] for b < nb: A entry (b,i,k) at b*asb + i*asi + k*ask (a_shared: one A for all b), B entry
  * (b,k,j) at b*bsb + k*bsk + j*bsj, C entry (b,i,j) at b*csb + i*csi + j*csj.  A aligned per row, B per
- * column, exact sum of products, one rounding.  Residue tables (device): nmod primes < 2^15, pw[t*L + l] =
- * 2^(32 l) mod p_t, inv[t] = (M/p_t)^-1 mod p_t, rp[t] = 1/p_t, Mp[t*(2L+2) + q] limbs of M/p_t, Mfull the
- * 2L+2 limbs of M = prod p_t (>= 2^(2(32L-2) + log2 K + 4)).  EB: scratch of nb*N int64.  Columns are
- * processed Nc at a time within `workspace` (b3d_rns_workspace_bytes).  Requires K <= 2^16, nb < 65536. */
-int b3d_rns_gemm(int L, int nmod, const uint16_t* primes, const uint32_t* pw, const uint16_t* inv, const double* rp,
-                 const uint32_t* Mp, const uint32_t* Mfull, int64_t nb, int64_t M, int64_t N, int64_t K, int a_shared,
+ * column, exact sum of products, one rounding.  Residue tables (device): nmod primes < 2^15; residue
+ * conversion weights w8 (int8, [2][n16][KB], n16 = nmod rounded up to 16, KB = 4L rounded up to 64):
+ * c = 2^(8 j) mod p_t balanced and split c = 256 w8[0][t][j] + w8[1][t][j], wcorr[t] = 128 sum_j c (int32);
+ * inv[t] = (M/p_t)^-1 mod p_t, rp[t] = 1/p_t with M = prod p_t >= 2^(2(32L-2) + log2 K + 4); CRT digits c8
+ * (int8, [CN][CK], CN = 4(2L+2)+2 rounded up to 16, CK = 2 nmod + 1 rounded up to 64): column 2t+1 the signed
+ * radix-256 digits of M/p_t, column 2t the same one byte up, column 2 nmod those of M.  EB: scratch of nb*N
+ * int64.  Columns are processed Nc at a time within `workspace` (b3d_rns_workspace_bytes).  Requires
+ * K <= 2^16, nb < 65536, nmod <= 255. */
+int b3d_rns_gemm(int L, int nmod, const uint16_t* primes, const int8_t* w8, const int32_t* wcorr, const uint16_t* inv,
+                 const double* rp, const int8_t* c8, int CK, int64_t nb, int64_t M, int64_t N, int64_t K, int a_shared,
                  const int32_t* Am, const int64_t* Ae, int64_t aps, int64_t asb, int64_t asi, int64_t ask,
                  const int32_t* Bm, const int64_t* Be, int64_t bps, int64_t bsb, int64_t bsk, int64_t bsj, int32_t* Cm,
                  int64_t* Ce, int64_t cps, int64_t csb, int64_t csi, int64_t csj, int64_t Nc, int64_t* EB,
