@@ -36,11 +36,7 @@ __global__ __launch_bounds__(256) void k_mom_stage(const int32_t* __restrict__ r
     if (a[L - 1] == 0u || b[L - 1] == 0u) {
       mp::zero(r);
     } else {
-      mp::cneg(a, mp::to_mag(b));               // sign of b folded into a
-      u32 acc[2 * L + 2];
-      mp::zero(acc);
-      mp::mac_signed(acc, a, b);
-      e = mp::norm<L, 2 * L + 2>(acc, re[slot[task] * N + c] + pe[n], r);
+      e = mp::mul_norm<L>(a, b, re[slot[task] * N + c] + pe[n], r);
     }
     const i64 o = woff + row[task] * N + c;
     mp::store(r, sm, sps, o);

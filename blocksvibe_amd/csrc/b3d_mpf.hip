@@ -85,11 +85,7 @@ __global__ __launch_bounds__(256) void k_mul(const int32_t* __restrict__ am, con
       mp::zero(r);
       e = B3D_EZ;
     } else {
-      mp::cneg(a, mp::to_mag(b));               // sign of b folded into a
-      u32 acc[2 * L + 2];
-      mp::zero(acc);
-      mp::mac_signed(acc, a, b);
-      e = mp::norm<L, 2 * L + 2>(acc, ae[i * sa] + be[i * sb], r);
+      e = mp::mul_norm<L>(a, b, ae[i * sa] + be[i * sb], r);
     }
     mp::store(r, cm, cps, i);
     ce[i] = e;

@@ -319,52 +319,78 @@ __device__ __forceinline__ void div_small(const u32 (&x)[L], u32 d, u32 (&q)[L +
   }
 }
 
-// acc += a * x, a an L-limb two's complement value, x an L-limb magnitude, acc NW >= 2L + 1 limbs
-// (wrapping).  Product scanning: the column sums of the unsigned product bits(a) * x are accumulated in
-// 96 bits (t0 + a_i x_j < 2^64 is one v_mad_u64_u32, the high half goes into t1:t2 with the carry flag)
-// and folded into acc one limb at a time; a negative a contributes bits(a) = a + 2^(32L), corrected by
-// subtracting x at limb offset L.
-template <int L, int NW>
-__device__ __forceinline__ void mac_signed(u32 (&acc)[NW], const u32 (&a)[L], const u32 (&x)[L]) {
-  u32 t0 = 0, t1 = 0, t2 = 0;
+// ---- 28-bit limbs for sums of products: a_i y_k < 2^56 in magnitude, so a column of up to 58 products
+// plus a normalised limb and a carry stays inside int64 and every product is one v_mad_i64_i32 (no carry
+// flags).  Signs are folded into the limbs of one operand (each limb negated).
+
+template <int L>
+struct R28 {
+  static constexpr int NL = (32 * L - 2 + 27) / 28;  // limbs of a magnitude below 2^(32L - 2)
+  static constexpr int NC = 2 * NL - 1;              // product columns
+};
+
+// y = sign * x in NL 28-bit limbs (x a magnitude of L limbs below 2^(32L-2); m = 0 or ~0 for the sign)
+template <int L, int NL>
+__device__ __forceinline__ void to28(const u32 (&x)[L], u32 m, int32_t (&y)[NL]) {
 #pragma unroll
-  for (int s = 0; s < 2 * L - 1; ++s) {
-#pragma unroll
-    for (int i = (s - L + 1 > 0 ? s - L + 1 : 0); i <= (s < L - 1 ? s : L - 1); ++i) {
-      const u64 v = (u64)a[i] * x[s - i] + (u64)t0;
-      t0 = (u32)v;
-      u32 c = 0;
-      t1 = addc(t1, (u32)(v >> 32), c);
-      t2 = addc(t2, 0u, c);
-    }
-    u32 c = 0;
-    acc[s] = addc(acc[s], t0, c);
-    t0 = addc(t1, 0u, c);
-    t1 = t2 + c;
-    t2 = 0;
+  for (int k = 0; k < NL; ++k) {
+    const int bit = 28 * k, w = bit >> 5, off = bit & 31;
+    const u32 lo = w < L ? x[w] : 0u;
+    const u32 hi = w + 1 < L ? x[w + 1] : 0u;
+    const u32 v = b3d_alignbit(hi, lo, (u32)off) & 0x0FFFFFFFu;
+    y[k] = (int32_t)((v ^ m) - m);
   }
-  {
-    u32 c = 0;
-    acc[2 * L - 1] = addc(acc[2 * L - 1], t0, c);
-    u32 hi = t1;
+}
+
+// acc += a * y with acc = NC normalised limbs in [0, 2^28) and a signed top limb acc[NC] (weight 2^(28 NC));
+// a: nonnegative 28-bit limbs, y: signed limbs.  Product scanning, one int64 column at a time.
+template <int NL>
+__device__ __forceinline__ void mac28(int32_t (&acc)[2 * NL], const int32_t (&a)[NL], const int32_t (&y)[NL]) {
+  constexpr int NC = 2 * NL - 1;
+  i64 carry = 0;
 #pragma unroll
-    for (int s = 2 * L; s < NW; ++s) {
-      acc[s] = addc(acc[s], hi, c);
-      hi = 0;
-    }
+  for (int c = 0; c < NC; ++c) {
+    i64 s = (i64)acc[c] + carry;
+#pragma unroll
+    for (int i = (c - NL + 1 > 0 ? c - NL + 1 : 0); i <= (c < NL - 1 ? c : NL - 1); ++i)
+      s += (i64)a[i] * (i64)y[c - i];
+    acc[c] = (int32_t)(s & 0x0FFFFFFF);
+    carry = s >> 28;
   }
-  const u32 m = sign_mask(a);                 // acc -= x * 2^(32 L) if a < 0
-  u32 b = 0;
+  acc[NC] += (int32_t)carry;
+}
+
+// the integer held by acc (NC limbs of 28 bits + signed top) as NW two's complement 32-bit limbs
+template <int NA, int NW>
+__device__ __forceinline__ void pack28(const int32_t (&acc)[NA], u32 (&W)[NW]) {
+  constexpr int NC = NA - 1;
+  constexpr int TB = 28 * NC, W0 = TB >> 5, OFF = TB & 31;
+  static_assert(W0 + 1 < NW, "top limb must fit the 32-bit accumulator");
 #pragma unroll
-  for (int s = L; s < NW; ++s) acc[s] = subb(acc[s], (s - L < L ? x[s - L] : 0u) & m, b);
+  for (int w = 0; w < NW; ++w) W[w] = 0u;
+#pragma unroll
+  for (int k = 0; k < NC; ++k) {
+    const int bit = 28 * k, w = bit >> 5, off = bit & 31;
+    const u32 v = (u32)acc[k];
+    W[w] |= v << off;
+    if (off > 4 && w + 1 < NW) W[w + 1] |= v >> (32 - off);
+  }
+  const i64 T = acc[NC];
+  const u64 t = (u64)(T * ((i64)1 << OFF));
+  W[W0] |= (u32)t;
+  W[W0 + 1] |= (u32)(t >> 32);
+  const u32 ext = T < 0 ? 0xFFFFFFFFu : 0u;
+#pragma unroll
+  for (int w = W0 + 2; w < NW; ++w) W[w] = ext;
 }
 
 // out = norm(sum_j A[j] * X[j]) over K = S terms: A pre-aligned (mantissas a_j, exponent ea), X entries
-// aligned here (over j).  Shared by both pole-step products.
+// aligned here (over j).  Shared by both pole-step products.  The exact sum is accumulated in 28-bit limbs.
 template <int L>
 __device__ __forceinline__ i64 small_dot(const int32_t* __restrict__ am, i64 aps, i64 abase, i64 ea,
                                          const int32_t* __restrict__ xm, const i64* __restrict__ xe, i64 xps,
                                          i64 xbase, i64 xstride, int S, u32 (&out)[L]) {
+  constexpr int NL = R28<L>::NL;
   i64 E = B3D_EZ;
   for (int j = 0; j < S; ++j) {
     const i64 idx = xbase + j * xstride;
@@ -377,8 +403,9 @@ __device__ __forceinline__ i64 small_dot(const int32_t* __restrict__ am, i64 aps
     mp::zero(out);
     return B3D_EZ;
   }
-  u32 acc[2 * L + 2];
-  mp::zero(acc);
+  int32_t acc[2 * NL];
+#pragma unroll
+  for (int c = 0; c < 2 * NL; ++c) acc[c] = 0;
   for (int j = 0; j < S; ++j) {
     const i64 idx = xbase + j * xstride;
     u32 x[L];
@@ -390,10 +417,31 @@ __device__ __forceinline__ i64 small_dot(const int32_t* __restrict__ am, i64 aps
     mp::shr(x, (int)sh);
     u32 a[L];
     mp::load(a, am, aps, abase + j);
-    mp::cneg(a, sx);                        // sign of x folded into a (|a| < 2^(32L-2): no overflow)
-    mac_signed(acc, a, x);
+    const u32 sa = mp::to_mag(a);
+    int32_t a28[NL], y28[NL];
+    to28(a, 0u, a28);
+    to28(x, sx ^ sa, y28);
+    mac28(acc, a28, y28);
   }
-  return mp::norm<L, 2 * L + 2>(acc, ea + E, out);
+  u32 W[2 * L + 2];
+  pack28(acc, W);
+  return mp::norm<L, 2 * L + 2>(W, ea + E, out);
+}
+
+// out = norm(a * b) for two's complement a, b of L limbs (both destroyed), exponent e of the product
+template <int L>
+__device__ __forceinline__ i64 mul_norm(u32 (&a)[L], u32 (&b)[L], i64 e, u32 (&out)[L]) {
+  constexpr int NL = R28<L>::NL;
+  const u32 sa = to_mag(a), sb = to_mag(b);
+  int32_t a28[NL], b28[NL], acc[2 * NL];
+#pragma unroll
+  for (int c = 0; c < 2 * NL; ++c) acc[c] = 0;
+  to28(a, 0u, a28);
+  to28(b, sa ^ sb, b28);
+  mac28(acc, a28, b28);
+  u32 W[2 * L + 2];
+  pack28(acc, W);
+  return norm<L, 2 * L + 2>(W, e, out);
 }
 
 // One Cauchy-step entry: h + sum_i x_i * 2 / (D - D_i) for column n of a group whose prime poles are the

@@ -38,11 +38,7 @@ template <int L> static int64_t mul_t(const uint32_t* a, int64_t ea, const uint3
     for (int k = 0; k < L; ++k) r[k] = 0;
     return B3D_EZ;
   }
-  mp::cneg(A, mp::to_mag(B));
-  u32 acc[2 * L + 2];
-  mp::zero(acc);
-  mp::mac_signed(acc, A, B);
-  const int64_t e = mp::norm<L, 2 * L + 2>(acc, ea + eb, R);
+  const int64_t e = mp::mul_norm<L>(A, B, ea + eb, R);
   copy_out(r, R);
   return e;
 }

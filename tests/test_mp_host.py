@@ -1,5 +1,5 @@
 """The gfx950 kernels' per-thread multiprecision helpers (blocksvibe_amd/csrc/mp.hpp), compiled for the
-host CPU (tests/host/mp_host.cpp), against the oracle bit for bit: add, mul (product scanning), norm at
+host CPU (tests/host/mp_host.cpp), against the oracle bit for bit: add, mul (28-bit limb products), norm at
 every accumulator width the kernels use, the FP64 one-limb division (every digit exact, divisors up to
 2^15 - 1), barrel shifts, Cauchy entries with h_inf, pole-step dot products and moment windows.  CPU only;
 the same source is what the GPU runs."""
@@ -153,7 +153,7 @@ def test_cauchy_entry(lib, L):
         assert got(r, e) == R.cauchy(xs, ds, L, hv[1]), trial
 
 
-@pytest.mark.parametrize("L", [4, 10, 34])
+@pytest.mark.parametrize("L", [2, 4, 10, 34, 50])
 def test_small_dot(lib, L):
     rng = random.Random(500 + L)
     for trial in range(300):
