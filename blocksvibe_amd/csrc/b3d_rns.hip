@@ -11,7 +11,7 @@
 // v_mfma_i32_16x16x64_i8 (int32 accumulation, exact for K < 2^17) give
 //   r_A r_B summed = 2^16 (h h) + 2^8 (h l + l h) + (l l).
 // Stages: (1) per-line exponent maxima, (2) aligned limbs -> residue digits (int8 planes, K contiguous,
-// padded to 64), (3) MFMA products reduced mod p -> uint16 residues, (4) CRT lift (y_t = c_t (M/p_t)^-1,
+// padded to 16), (3) MFMA products reduced mod p -> uint16 residues, (4) CRT lift (y_t = c_t (M/p_t)^-1,
 // C = sum_t y_t M/p_t - round(sum_t y_t / p_t) M, exact in 2L + 2 limbs) and normalisation.
 #include "common.hpp"
 #include "mp.hpp"
@@ -63,7 +63,8 @@ __global__ __launch_bounds__(256) void k_line_max(const int32_t* __restrict__ m,
 //   phase 2  per 16 primes: 4 entry tiles x KB/64 steps x 2 digit halves of v_mfma_i32_16x16x64_i8
 //   phase 3  Y = 256 Yh + Yl + wcorr mod p, sign, balanced residue -> int8 digits (h, l), staged in LDS
 //            and written as 64 contiguous bytes per output plane.
-// Output planes out[(t*2 + d) * plane + entry], d = 0 high digit, 1 low digit.
+// Output planes out[(t*2 + d) * plane + entry], d = 0 high digit, 1 low digit; the K dimension is padded to
+// Kp = K rounded up to 16 (zero digits), so the number of entries is a multiple of 16.
 template <int L>
 __global__ __launch_bounds__(64) void k_to_residues(const int32_t* __restrict__ m, const i64* __restrict__ e, i64 ps,
                                                     i64 nb, i64 lines, i64 K, i64 sb, i64 sl, i64 sk,
@@ -81,7 +82,7 @@ __global__ __launch_bounds__(64) void k_to_residues(const int32_t* __restrict__ 
   __shared__ u32 sgn[64];
   const int lane = threadIdx.x;
   const int r = lane & 15, g = lane >> 4;
-  const i64 total = nb * (i64)Lp * Kp;  // a multiple of 64 (Kp is)
+  const i64 total = nb * (i64)Lp * Kp;  // a multiple of 16 (Kp is)
   for (i64 t0 = (i64)blockIdx.x * 64; t0 < total; t0 += (i64)gridDim.x * 64) {
     // phase 1
     {
@@ -90,7 +91,7 @@ __global__ __launch_bounds__(64) void k_to_residues(const int32_t* __restrict__ 
       u32 x[L];
       mp::zero(x);
       u32 sm = 0;
-      if (line < lines && k < K) {
+      if (t < total && line < lines && k < K) {
         const i64 idx = b * sb + line * sl + k * sk;
         mp::load(x, m, ps, idx);
         if (x[L - 1] != 0u) {
@@ -156,8 +157,8 @@ __global__ __launch_bounds__(64) void k_to_residues(const int32_t* __restrict__ 
         if (tq < nmod) {
           const v4i* src = (const v4i*)(stage + pd * 64 + half * 32);
           v4i* dst = (v4i*)(out + (i64)(tq * 2 + (pd & 1)) * plane + t0 + half * 32);
-          dst[0] = src[0];
-          dst[1] = src[1];
+          if (t0 + half * 32 < total) dst[0] = src[0];          // the tile's tail: 16-byte groups
+          if (t0 + half * 32 + 16 < total) dst[1] = src[1];
         }
       }
       __syncthreads();
@@ -186,12 +187,15 @@ __global__ __launch_bounds__(256) void k_rns_mma(const int8_t* __restrict__ A8, 
   const int r = lane & 15, g = lane >> 4;
   v4i hh = {0, 0, 0, 0}, mid = {0, 0, 0, 0}, ll = {0, 0, 0, 0};
   for (int k0 = 0; k0 < Kp; k0 += 64) {
+    // K is padded to 16 only: the 16-byte groups beyond Kp of the last step are zero
+    const bool kin = k0 + g * 16 < Kp;
     const i64 ao = (i64)(row0 + r) * Kp + k0 + g * 16;
     const i64 bo = (i64)(col0 + r) * Kp + k0 + g * 16;
-    const v4i ah = *(const v4i*)(Ah + ao);
-    const v4i al = *(const v4i*)(Al + ao);
-    const v4i bh = *(const v4i*)(Bh + bo);
-    const v4i bl = *(const v4i*)(Bl + bo);
+    const v4i z = {0, 0, 0, 0};
+    const v4i ah = kin ? *(const v4i*)(Ah + ao) : z;
+    const v4i al = kin ? *(const v4i*)(Al + ao) : z;
+    const v4i bh = kin ? *(const v4i*)(Bh + bo) : z;
+    const v4i bl = kin ? *(const v4i*)(Bl + bo) : z;
     hh = __builtin_amdgcn_mfma_i32_16x16x64_i8(ah, bh, hh, 0, 0, 0);
     mid = __builtin_amdgcn_mfma_i32_16x16x64_i8(ah, bl, mid, 0, 0, 0);
     mid = __builtin_amdgcn_mfma_i32_16x16x64_i8(al, bh, mid, 0, 0, 0);
@@ -216,11 +220,11 @@ __global__ __launch_bounds__(256) void k_rns_mma(const int8_t* __restrict__ A8, 
 // is one int8 GEMM against the constant signed radix-256 digits of M/p_t (row l_t), of 256 M/p_t (row h_t)
 // and of M (row -q): c8[byte column][k], k < CK.  Byte columns run along the MFMA rows, so each lane holds
 // the sums of four consecutive byte columns = one 32-bit limb's worth (|sum| < 2^48 in int64).  One wave
-// lifts 16 outputs per pass:
-//   phase 1  64 lanes: residues -> balanced y digits (output r = lane % 16, primes t = lane / 16 mod 4)
-//            into LDS, q from the per-lane partial sums of y_t / p_t
-//   phase 2  per 16 byte columns: CK/64 steps of v_mfma_i32_16x16x64_i8 -> limb partial sums into LDS
-//   phase 3  lanes 0..15: carry propagation into 2L + 2 limbs, one rounding, store.
+// lifts 64 outputs (4 MFMA column tiles of 16) per pass:
+//   phase 1  lane = output: residues (one coalesced row per prime) -> y digits and -q into an LDS row,
+//            then each lane loads its MFMA operand fragments (output r of tile ot, bytes 64 ks + 16 g ..)
+//   phase 2  per 16 byte columns: CK/64 steps of v_mfma_i32_16x16x64_i8 per tile -> limb sums into LDS
+//   phase 3  lane = output: carry propagation into 2L + 2 limbs, one rounding, store.
 // Output entry (b, i, j) at b*csb + i*csi + j*csj (+ column offset j0).
 template <int L>
 constexpr int crt_ckmax() {
@@ -237,59 +241,75 @@ __global__ __launch_bounds__(64) void k_crt(const uint16_t* __restrict__ Cr, i64
   constexpr int LW = 2 * L + 2;
   constexpr int CN = (4 * LW + 2 + 15) / 16 * 16;
   constexpr int NRT = CN / 16;
-  constexpr int VW = 4 * NRT + 1;
-  constexpr int CKMAX = crt_ckmax<L>();
-  constexpr int YROW = CKMAX + 16;
-  __shared__ __align__(16) int8_t yv[16 * YROW];
-  __shared__ i64 V[16 * VW];
-  __shared__ double qpart[64];
+  constexpr int VW = LW + 1;
+  constexpr int NKS = crt_ckmax<L>() / 64;
+  constexpr int YROW = crt_ckmax<L>() + 16;
+  static_assert(64 * YROW <= 64 * VW * 8, "digit rows must fit the limb-sum buffer they share");
+  __shared__ i64 V[64 * VW];
+  int8_t* yv = (int8_t*)V;
   const int lane = threadIdx.x, r = lane & 15, g = lane >> 4;
   const i64 total = nb * (i64)M * N;
   const int nks = CK / 64;
-  for (i64 o0 = (i64)blockIdx.x * 16; o0 < total; o0 += (i64)gridDim.x * 16) {
-    const i64 o = o0 + r;
-    const bool live = o < total;
-    // phase 1
-    double qs = 0.0;
-    int8_t* yrow = yv + r * YROW;
-    for (int t = g; t < nmod; t += 4) {
-      int yb = 0;
-      if (live) {
-        const uint32_t p = primes[t];
-        const double pd = (double)p;
-        const double v = (double)((uint32_t)Cr[(i64)t * total + o] * (uint32_t)inv[t]);   // < 2^30
-        double y = fma(-floor(v * rp[t]), pd, v);
-        y = y < 0.0 ? y + pd : y;
-        y = y >= pd ? y - pd : y;
-        yb = (int)y;
-        if (yb > (int)(p >> 1)) yb -= (int)p;
-        qs += (double)yb * rp[t];
+  for (i64 o0 = (i64)blockIdx.x * 64; o0 < total; o0 += (i64)gridDim.x * 64) {
+    // phase 1: lane = output (coalesced residue loads), digits into LDS rows, then each lane's operand
+    // fragments (output r of tile ot, bytes 64 ks + 16 g ..) into registers
+    {
+      const i64 o = o0 + lane;
+      int8_t* yrow = yv + lane * YROW;
+      double qs = 0.0;
+      if (o < total) {
+        for (int t = 0; t < nmod; ++t) {
+          const uint32_t p = primes[t];
+          const double pd = (double)p;
+          const double v = (double)((uint32_t)Cr[(i64)t * total + o] * (uint32_t)inv[t]);   // < 2^30
+          double y = fma(-floor(v * rp[t]), pd, v);
+          y = y < 0.0 ? y + pd : y;
+          y = y >= pd ? y - pd : y;
+          int yb = (int)y;
+          if (yb > (int)(p >> 1)) yb -= (int)p;
+          qs += (double)yb * rp[t];
+          const int h = (yb + 128) >> 8;
+          *(uint16_t*)(yrow + 2 * t) = (uint16_t)(((u32)h & 0xFFu) | (((u32)(yb - 256 * h) & 0xFFu) << 8));
+        }
+      } else {
+        for (int t = 0; t < nmod; ++t) *(uint16_t*)(yrow + 2 * t) = 0;
       }
-      const int h = (yb + 128) >> 8;
-      yrow[2 * t] = (int8_t)h;
-      yrow[2 * t + 1] = (int8_t)(yb - 256 * h);
+      yrow[2 * nmod] = (int8_t)(-(int)rint(qs));
+      for (int k = 2 * nmod + 1; k < CK; ++k) yrow[k] = 0;
     }
-    for (int k = 2 * nmod + 1 + g; k < CK; k += 4) yrow[k] = 0;
-    qpart[lane] = qs;
     __syncthreads();
-    if (g == 0) {
-      const double sq = qpart[r] + qpart[r + 16] + qpart[r + 32] + qpart[r + 48];
-      yrow[2 * nmod] = (int8_t)(-(int)rint(sq));
-    }
+    v4i Bf[4][NKS];
+#pragma unroll
+    for (int ot = 0; ot < 4; ++ot)
+#pragma unroll
+      for (int ks = 0; ks < NKS; ++ks)
+        Bf[ot][ks] = ks < nks ? *(const v4i*)(yv + (16 * ot + r) * YROW + ks * 64 + g * 16) : v4i{0, 0, 0, 0};
     __syncthreads();
     // phase 2
     for (int rt = 0; rt < NRT; ++rt) {
-      v4i acc = {0, 0, 0, 0};
-      for (int ks = 0; ks < nks; ++ks) {
-        const v4i a = *(const v4i*)(c8 + (i64)(rt * 16 + r) * CK + ks * 64 + g * 16);
-        const v4i b = *(const v4i*)(yrow + ks * 64 + g * 16);
-        acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, acc, 0, 0, 0);
+      v4i acc[4];
+#pragma unroll
+      for (int ot = 0; ot < 4; ++ot) acc[ot] = v4i{0, 0, 0, 0};
+#pragma unroll
+      for (int ks = 0; ks < NKS; ++ks) {
+        if (ks < nks) {
+          const v4i a = *(const v4i*)(c8 + (i64)(rt * 16 + r) * CK + ks * 64 + g * 16);
+#pragma unroll
+          for (int ot = 0; ot < 4; ++ot) acc[ot] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, Bf[ot][ks], acc[ot], 0, 0, 0);
+        }
       }
-      V[r * VW + 4 * rt + g] = (i64)acc[0] + (i64)acc[1] * 256 + (i64)acc[2] * 65536 + (i64)acc[3] * 16777216;
+      const int k = 4 * rt + g;
+      if (k < LW) {
+#pragma unroll
+        for (int ot = 0; ot < 4; ++ot)
+          V[(16 * ot + r) * VW + k] = (i64)acc[ot][0] + (i64)acc[ot][1] * 256 + (i64)acc[ot][2] * 65536 +
+                                      (i64)acc[ot][3] * 16777216;
+      }
     }
     __syncthreads();
     // phase 3
-    if (g == 0 && live) {
+    const i64 o = o0 + lane;
+    if (o < total) {
       const i64 j = o % N, i = (o / N) % M, b = o / ((i64)N * M);
       const i64 ea = EA[(a_shared ? 0 : b) * M + i];
       const i64 eb = EB[b * Nfull + j0 + j];
@@ -302,7 +322,7 @@ __global__ __launch_bounds__(64) void k_crt(const uint16_t* __restrict__ Cr, i64
         i64 carry = 0;
 #pragma unroll
         for (int k = 0; k < LW; ++k) {
-          const i64 v = V[r * VW + k] + carry;
+          const i64 v = V[lane * VW + k] + carry;
           acc[k] = (u32)v;
           carry = v >> 32;   // arithmetic: floor division by 2^32
         }
@@ -319,7 +339,7 @@ __global__ __launch_bounds__(64) void k_crt(const uint16_t* __restrict__ Cr, i64
 extern "C" {
 
 int64_t b3d_rns_workspace_bytes(int nmod, int64_t nbA, int64_t nb, int64_t M, int64_t Nc, int64_t K) {
-  const int64_t Mp = (M + 15) / 16 * 16, Np = (Nc + 63) / 64 * 64, Kp = (K + 63) / 64 * 64;
+  const int64_t Mp = (M + 15) / 16 * 16, Np = (Nc + 63) / 64 * 64, Kp = (K + 15) / 16 * 16;
   int64_t bytes = 0;
   bytes += 2 * (int64_t)nmod * nbA * Mp * Kp;  // A digits
   bytes += 2 * (int64_t)nmod * nb * Np * Kp;   // B digits (one chunk)
@@ -355,7 +375,7 @@ int b3d_rns_gemm(int L, int nmod, const uint16_t* primes, const int8_t* w8, cons
     return 2;
   }
   hipStream_t s = (hipStream_t)stream;
-  const int Mpad = (int)((M + 15) / 16 * 16), Npad = (int)((Nc + 63) / 64 * 64), Kpad = (int)((K + 63) / 64 * 64);
+  const int Mpad = (int)((M + 15) / 16 * 16), Npad = (int)((Nc + 63) / 64 * 64), Kpad = (int)((K + 15) / 16 * 16);
   char* w = (char*)workspace;
   int8_t* A8 = (int8_t*)w;
   w += 2 * (int64_t)nmod * nbA * Mpad * Kpad;
@@ -384,7 +404,7 @@ int b3d_rns_gemm(int L, int nmod, const uint16_t* primes, const int8_t* w8, cons
     dim3 grid((unsigned)(((Mpad + 15) / 16) * ((Np + 63) / 64)), (unsigned)nmod, (unsigned)nb);
     k_rns_mma<<<grid, 256, 0, s>>>(A8, aplane, a_shared, B8, nb * (int64_t)Np * Kpad, Mpad, Np, Kpad, (int)M,
                                    (int)nc, primes, rp, Cr, nb);
-    B3D_DISPATCH_L(L, k_crt<LL><<<grid_for(nb * M * nc, 16), 64, 0, s>>>(
+    B3D_DISPATCH_L(L, k_crt<LL><<<grid_for(nb * M * nc, 64), 64, 0, s>>>(
                           Cr, nb, (int)M, (int)nc, nmod, primes, inv, rp, c8, CK, EA, a_shared, EB, j0, N, Cm, Ce,
                           cps, csb, csi, csj));
     if (b3d::check_launch("b3d_rns_gemm (chunk)")) return 1;

@@ -87,12 +87,28 @@ __device__ __forceinline__ bool is_zero(const u32 (&a)[N]) {
   return o == 0u;
 }
 
+// bit length.  The values normalised here have their top bit within the highest few limbs, so the top four
+// limbs are searched with selects and the rest only when some lane of the wave has a nonzero value below
+// them (one OR per limb decides that).
 template <int N>
 __device__ __forceinline__ int bitlen(const u32 (&a)[N]) {
+  constexpr int H = N < 4 ? N : 4;
   int bl = 0;
 #pragma unroll
-  for (int k = 0; k < N; ++k)
+  for (int k = N - H; k < N; ++k)
     if (a[k]) bl = 32 * k + 32 - __builtin_clz(a[k]);
+  if (N > H) {
+    u32 low = 0;
+#pragma unroll
+    for (int k = 0; k < N - H; ++k) low |= a[k];
+    if (B3D_ANY(bl == 0 && low != 0u)) {
+      int b2 = 0;
+#pragma unroll
+      for (int k = 0; k < N - H; ++k)
+        if (a[k]) b2 = 32 * k + 32 - __builtin_clz(a[k]);
+      bl = bl ? bl : b2;
+    }
+  }
   return bl;
 }
 

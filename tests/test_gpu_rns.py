@@ -31,6 +31,9 @@ def mat(rng, L, r, c, spread):
     (34, 2, 12, 61, 150, True, None),
     (34, 3, 6, 136, 80, False, None),
     (50, 1, 4, 40, 70, False, None),
+    (34, 5, 12, 12, 300, False, None),           # the derivative stages' K = R + 1
+    (18, 2, 9, 24, 64, True, None),
+    (26, 1, 16, 17, 20, True, None),
 ])
 def test_rns_gemm_bit_exact(L, nb, M, K, N, shared, budget):
     from blocksvibe_amd import mpf, rns
