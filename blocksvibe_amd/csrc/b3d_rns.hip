@@ -166,8 +166,8 @@ __global__ __launch_bounds__(64) void k_to_residues(const int32_t* __restrict__ 
   }
 }
 
-// (3) per-prime products on MFMA: each wave one 16 x 16 output tile, a block of 4 waves covers 16 x 64.
-// Residues out: Cr[(t * nb + b) * M * Nc + i * Nc + j] (uint16).
+// (3) per-prime products on MFMA: each wave one 16 x 16 output tile, a block of 4 waves covers 16 x 64,
+// staged in LDS and written by rows.  Residues out: Cr[(t * nb + b) * M * Nc + i * Nc + j] (uint16).
 __global__ __launch_bounds__(256) void k_rns_mma(const int8_t* __restrict__ A8, i64 aplane, int a_shared,
                                                  const int8_t* __restrict__ B8, i64 bplane, int Mp, int Np, int Kp,
                                                  int M, int N, const uint16_t* __restrict__ primes,
@@ -178,7 +178,7 @@ __global__ __launch_bounds__(256) void k_rns_mma(const int8_t* __restrict__ A8, 
   const int ntn = (Np + 63) / 64;
   const int tile_m = blockIdx.x / ntn, tile_n = blockIdx.x % ntn;
   const int row0 = tile_m * 16, col0 = tile_n * 64 + wave * 16;
-  if (col0 >= Np) return;
+  __shared__ uint16_t st[16][64];   // the block's 16 x 64 residues, stored by rows of 128 bytes
   const i64 ab = a_shared ? 0 : b;
   const int8_t* Ah = A8 + (i64)(tm * 2 + 0) * aplane + (ab * Mp) * Kp;
   const int8_t* Al = A8 + (i64)(tm * 2 + 1) * aplane + (ab * Mp) * Kp;
@@ -186,7 +186,7 @@ __global__ __launch_bounds__(256) void k_rns_mma(const int8_t* __restrict__ A8, 
   const int8_t* Bl = B8 + (i64)(tm * 2 + 1) * bplane + (b * Np) * Kp;
   const int r = lane & 15, g = lane >> 4;
   v4i hh = {0, 0, 0, 0}, mid = {0, 0, 0, 0}, ll = {0, 0, 0, 0};
-  for (int k0 = 0; k0 < Kp; k0 += 64) {
+  for (int k0 = 0; k0 < (col0 < Np ? Kp : 0); k0 += 64) {
     // K is padded to 16 only: the 16-byte groups beyond Kp of the last step are zero
     const bool kin = k0 + g * 16 < Kp;
     const i64 ao = (i64)(row0 + r) * Kp + k0 + g * 16;
@@ -203,13 +203,19 @@ __global__ __launch_bounds__(256) void k_rns_mma(const int8_t* __restrict__ A8, 
   }
   const uint32_t p = primes[tm];
   const double rpt = rp[tm];
-  const int col = col0 + r;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    const int row = row0 + 4 * g + i;
-    if (row < M && col < N) {
-      const int64_t v = (int64_t)hh[i] * 65536 + (int64_t)mid[i] * 256 + (int64_t)ll[i];
-      Cr[((i64)tm * nb + b) * M * N + (i64)row * N + col] = (uint16_t)mod_signed(v, p, rpt);
+    const int64_t v = (int64_t)hh[i] * 65536 + (int64_t)mid[i] * 256 + (int64_t)ll[i];
+    st[4 * g + i][wave * 16 + r] = (uint16_t)mod_signed(v, p, rpt);
+  }
+  __syncthreads();
+  {
+    const int row = row0 + (threadIdx.x >> 4), cl = (threadIdx.x & 15) * 4, cb = tile_n * 64;
+    if (row < M) {
+      uint16_t* dst = Cr + ((i64)tm * nb + b) * M * N + (i64)row * N + cb;
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        if (cb + cl + c < N) dst[cl + c] = st[threadIdx.x >> 4][cl + c];
     }
   }
 }
@@ -258,18 +264,27 @@ __global__ __launch_bounds__(64) void k_crt(const uint16_t* __restrict__ Cr, i64
       int8_t* yrow = yv + lane * YROW;
       double qs = 0.0;
       if (o < total) {
-        for (int t = 0; t < nmod; ++t) {
-          const uint32_t p = primes[t];
-          const double pd = (double)p;
-          const double v = (double)((uint32_t)Cr[(i64)t * total + o] * (uint32_t)inv[t]);   // < 2^30
-          double y = fma(-floor(v * rp[t]), pd, v);
-          y = y < 0.0 ? y + pd : y;
-          y = y >= pd ? y - pd : y;
-          int yb = (int)y;
-          if (yb > (int)(p >> 1)) yb -= (int)p;
-          qs += (double)yb * rp[t];
-          const int h = (yb + 128) >> 8;
-          *(uint16_t*)(yrow + 2 * t) = (uint16_t)(((u32)h & 0xFFu) | (((u32)(yb - 256 * h) & 0xFFu) << 8));
+        for (int t0 = 0; t0 < nmod; t0 += 16) {
+          uint32_t c[16];   // 16 loads in flight per lane
+#pragma unroll
+          for (int u = 0; u < 16; ++u) c[u] = t0 + u < nmod ? (uint32_t)Cr[(i64)(t0 + u) * total + o] : 0u;
+#pragma unroll
+          for (int u = 0; u < 16; ++u) {
+            const int t = t0 + u;
+            if (t < nmod) {
+              const uint32_t p = primes[t];
+              const double pd = (double)p;
+              const double v = (double)(c[u] * (uint32_t)inv[t]);   // < 2^30
+              double y = fma(-floor(v * rp[t]), pd, v);
+              y = y < 0.0 ? y + pd : y;
+              y = y >= pd ? y - pd : y;
+              int yb = (int)y;
+              if (yb > (int)(p >> 1)) yb -= (int)p;
+              qs += (double)yb * rp[t];
+              const int h = (yb + 128) >> 8;
+              *(uint16_t*)(yrow + 2 * t) = (uint16_t)(((u32)h & 0xFFu) | (((u32)(yb - 256 * h) & 0xFFu) << 8));
+            }
+          }
         }
       } else {
         for (int t = 0; t < nmod; ++t) *(uint16_t*)(yrow + 2 * t) = 0;
