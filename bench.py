@@ -194,7 +194,8 @@ def serving_stream(seconds, seed=7):
     return {"requests": len(reqs), "rate_per_s": rate, "capacity_per_s": cap, "p50_ms": q(0.5) * 1e3,
             "p99_ms": q(0.99) * 1e3, "blocks_per_s": sum(r.blocks for r in reqs) / wall,
             "mean_batch": float(np.mean(srv.batch_sizes)) if srv.batch_sizes else 0.0, "warmup_s": warm_s,
-            "operators": len(pool)}
+            "operators": len(pool), "graph_replays": srv.replays, "graphs": len(srv.graphs),
+            "graph_pool_mib": srv.graph_bytes / 2 ** 20}
 
 
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_cauchy_nmax6_1024bit_pmc.txt")
@@ -326,6 +327,9 @@ def main():
     if args.serve > 0:
         res["serving"] = serving_stream(args.serve)
         print(f"# serving: {res['serving']}", file=sys.stderr, flush=True)
+    from blocksvibe_amd import rns
+    tiles = rns.tuned()
+    res["rns_tiles"] = {f"wm{t}": sum(1 for v in tiles.values() if v == t) for t in rns.TILES}
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(runs)
     print(json.dumps(res), flush=True)

@@ -166,19 +166,23 @@ __global__ __launch_bounds__(64) void k_to_residues(const int32_t* __restrict__ 
   }
 }
 
-// (3) per-prime products on MFMA: each wave one 16 x 16 output tile, a block of 4 waves covers 16 x 64,
-// staged in LDS and written by rows.  Residues out: Cr[(t * nb + b) * M * Nc + i * Nc + j] (uint16).
+// (3) per-prime products on MFMA: each wave one 16 x 16 output tile; the 4 waves of a block cover
+// (16 wm) x (64 / wm) outputs (wm = 1, 2 or 4, chosen by the caller's tuner), staged in LDS and written by
+// rows.  Residues out: Cr[(t * nb + b) * M * Nc + i * Nc + j] (uint16).
 __global__ __launch_bounds__(256) void k_rns_mma(const int8_t* __restrict__ A8, i64 aplane, int a_shared,
                                                  const int8_t* __restrict__ B8, i64 bplane, int Mp, int Np, int Kp,
                                                  int M, int N, const uint16_t* __restrict__ primes,
-                                                 const double* __restrict__ rp, uint16_t* __restrict__ Cr, i64 nb) {
+                                                 const double* __restrict__ rp, uint16_t* __restrict__ Cr, i64 nb,
+                                                 int wm) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wn = 4 / wm, bw = 16 * wn;
   const int tm = blockIdx.y;
   const i64 b = blockIdx.z;
-  const int ntn = (Np + 63) / 64;
+  const int ntn = (Np + bw - 1) / bw;
   const int tile_m = blockIdx.x / ntn, tile_n = blockIdx.x % ntn;
-  const int row0 = tile_m * 16, col0 = tile_n * 64 + wave * 16;
-  __shared__ uint16_t st[16][64];   // the block's 16 x 64 residues, stored by rows of 128 bytes
+  const int wi = wave / wn, wj = wave % wn;
+  const int row0 = tile_m * 16 * wm + 16 * wi, col0 = tile_n * bw + 16 * wj;
+  __shared__ uint16_t st[64][64];   // the block's residues, stored by rows
   const i64 ab = a_shared ? 0 : b;
   const int8_t* Ah = A8 + (i64)(tm * 2 + 0) * aplane + (ab * Mp) * Kp;
   const int8_t* Al = A8 + (i64)(tm * 2 + 1) * aplane + (ab * Mp) * Kp;
@@ -186,7 +190,7 @@ __global__ __launch_bounds__(256) void k_rns_mma(const int8_t* __restrict__ A8, 
   const int8_t* Bl = B8 + (i64)(tm * 2 + 1) * bplane + (b * Np) * Kp;
   const int r = lane & 15, g = lane >> 4;
   v4i hh = {0, 0, 0, 0}, mid = {0, 0, 0, 0}, ll = {0, 0, 0, 0};
-  for (int k0 = 0; k0 < (col0 < Np ? Kp : 0); k0 += 64) {
+  for (int k0 = 0; k0 < (col0 < Np && row0 < Mp ? Kp : 0); k0 += 64) {
     // K is padded to 16 only: the 16-byte groups beyond Kp of the last step are zero
     const bool kin = k0 + g * 16 < Kp;
     const i64 ao = (i64)(row0 + r) * Kp + k0 + g * 16;
@@ -206,16 +210,18 @@ __global__ __launch_bounds__(256) void k_rns_mma(const int8_t* __restrict__ A8, 
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int64_t v = (int64_t)hh[i] * 65536 + (int64_t)mid[i] * 256 + (int64_t)ll[i];
-    st[4 * g + i][wave * 16 + r] = (uint16_t)mod_signed(v, p, rpt);
+    st[16 * wi + 4 * g + i][16 * wj + r] = (uint16_t)mod_signed(v, p, rpt);
   }
   __syncthreads();
   {
-    const int row = row0 + (threadIdx.x >> 4), cl = (threadIdx.x & 15) * 4, cb = tile_n * 64;
+    const int tpr = 4 * wn;   // threads per row of the block tile
+    const int rl = threadIdx.x / tpr, cl = (threadIdx.x % tpr) * 4;
+    const int row = tile_m * 16 * wm + rl, cb = tile_n * bw;
     if (row < M) {
       uint16_t* dst = Cr + ((i64)tm * nb + b) * M * N + (i64)row * N + cb;
 #pragma unroll
       for (int c = 0; c < 4; ++c)
-        if (cb + cl + c < N) dst[cl + c] = st[threadIdx.x >> 4][cl + c];
+        if (cb + cl + c < N) dst[cl + c] = st[rl][cl + c];
     }
   }
 }
@@ -367,8 +373,8 @@ int b3d_rns_gemm(int L, int nmod, const uint16_t* primes, const int8_t* w8, cons
                  const double* rp, const int8_t* c8, int CK, int64_t nb, int64_t M, int64_t N, int64_t K, int a_shared,
                  const int32_t* Am, const int64_t* Ae, int64_t aps, int64_t asb, int64_t asi, int64_t ask,
                  const int32_t* Bm, const int64_t* Be, int64_t bps, int64_t bsb, int64_t bsk, int64_t bsj, int32_t* Cm,
-                 int64_t* Ce, int64_t cps, int64_t csb, int64_t csi, int64_t csj, int64_t Nc, int64_t* EB,
-                 void* workspace, int64_t ws_bytes, void* stream) {
+                 int64_t* Ce, int64_t cps, int64_t csb, int64_t csi, int64_t csj, int64_t Nc, int tile_wm,
+                 int64_t* EB, void* workspace, int64_t ws_bytes, void* stream) {
   if (nb <= 0 || M <= 0 || N <= 0) return 0;
   if (K <= 0 || K > (1 << 16)) {
     b3d::set_error("b3d_rns_gemm: need 0 < K <= 2^16");
@@ -381,6 +387,10 @@ int b3d_rns_gemm(int L, int nmod, const uint16_t* primes, const int8_t* w8, cons
   const int ckmax = (2 * ((2 * (32 * L - 2) + 20) / 14 + 2) + 1 + 63) / 64 * 64;   // crt_ckmax<L>()
   if (nmod <= 0 || nmod > 255 || CK % 64 != 0 || CK < 2 * nmod + 1 || CK > ckmax) {
     b3d::set_error("b3d_rns_gemm: CRT digit table does not fit (need 2 nmod + 1 <= CK <= bound, CK % 64 == 0)");
+    return 2;
+  }
+  if (tile_wm != 1 && tile_wm != 2 && tile_wm != 4) {
+    b3d::set_error("b3d_rns_gemm: tile_wm must be 1, 2 or 4");
     return 2;
   }
   if (Nc <= 0 || Nc > N) Nc = N;
@@ -416,9 +426,10 @@ int b3d_rns_gemm(int L, int nmod, const uint16_t* primes, const int8_t* w8, cons
     B3D_DISPATCH_L(L, k_to_residues<LL><<<grid_for(nb * (int64_t)Np * Kpad, 64), 64, 0, s>>>(
                           Bm + j0 * bsj, Be + j0 * bsj, bps, nb, nc, K, bsb, bsj, bsk, EB + j0, N, Np, Kpad, nmod,
                           primes, w8, wcorr, rp, B8, nb * (int64_t)Np * Kpad));
-    dim3 grid((unsigned)(((Mpad + 15) / 16) * ((Np + 63) / 64)), (unsigned)nmod, (unsigned)nb);
+    const int bh = 16 * tile_wm, bw = 64 / tile_wm;
+    dim3 grid((unsigned)(((Mpad + bh - 1) / bh) * ((Np + bw - 1) / bw)), (unsigned)nmod, (unsigned)nb);
     k_rns_mma<<<grid, 256, 0, s>>>(A8, aplane, a_shared, B8, nb * (int64_t)Np * Kpad, Mpad, Np, Kpad, (int)M,
-                                   (int)nc, primes, rp, Cr, nb);
+                                   (int)nc, primes, rp, Cr, nb, tile_wm);
     B3D_DISPATCH_L(L, k_crt<LL><<<grid_for(nb * M * nc, 64), 64, 0, s>>>(
                           Cr, nb, (int)M, (int)nc, nmod, primes, inv, rp, c8, CK, EA, a_shared, EB, j0, N, Cm, Ce,
                           cps, csb, csi, csj));

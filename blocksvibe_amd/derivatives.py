@@ -15,6 +15,7 @@ Pole shifting (kept_pole_order < order) replaces the unprotected poles by the ke
 S a, S_ki = (y'_k / y_i)^h l_k(y_i) (closed form of blocks_3d's A^-1 V, Lagrange basis l_k on the kept poles,
 h = size // 2), folded into the numerator matrix as exact rationals, rounded once.
 """
+import copy
 from fractions import Fraction
 from math import comb, lcm
 
@@ -124,6 +125,8 @@ class Derivs:
         self.NP = max(1, max(len(p) for p in self.poles))
         self._levels()
         self._constants()
+        self._finish_consts()
+        self._orders = {R: self}
 
     # --------------------------------------------------------------------------------------- bookkeeping
     def _levels(self):
@@ -272,19 +275,11 @@ class Derivs:
         Sm = self.plan.Smax
         S2, B, N = Sm * Sm, self.B, self.N
         Nc = 2 * N
-        dev = mpf.dev()
-        jidx = torch.tensor([(tj - self.plan.tj12 % 2) // 2 for tj in self.tjs], device=dev)
+        jidx, cidx, hidx, gidx, cb = self._fc
         H = mpf.MPF(h_inf.m[:, :, :order + 1][:, :, :, jidx], h_inf.e[:, :order + 1][:, :, jidx])  # (2,O1,Jo,S2,B)
         H = H.permute(1, 2, 0, 3, 4).reshape(order + 1, Jo * Nc)
         c = rns.gemm(self.Wh, H).reshape(R1, Jo, Nc)                                  # c[d]
         mom = self.mom.reshape(2, Jo, NP, N, R1).permute(1, 2, 4, 0, 3)                # (Jo, NP, R1, 2, N)
-        # index tables of the polynomial part
-        te = [(t, e) for t in range(R1) for e in range(R1)]
-        cidx = torch.tensor([e - t if t <= e else R1 for t, e in te], device=dev)
-        hidx = torch.tensor([e * R1 + (e - t) if t <= e else R1 * R1 for t, e in te], device=dev)
-        cbv = [mpf.norm_int(comb(e, t), 0, L) if t <= e else (0, mpf.EZ) for t, e in te]
-        gidx = torch.tensor([(s - 1 - d) * R1 + d if 1 <= s and d <= s - 1 else Rw * R1
-                             for s in range(R1) for d in range(R1)], device=dev)
         chunk = chunk or Jo
         outs = []
         for j0 in range(0, Jo, chunk):
@@ -302,7 +297,6 @@ class Derivs:
             Hm = rns.gemm(self.Cb, Gs).reshape(R1, J, R1, Nc)                           # (e, J, s, Nc)
             Hz = mpf.cat([Hm.permute(0, 2, 1, 3).reshape(R1 * R1, J, Nc), mpf.zeros((1, J, Nc), L)], 0)
             cz = mpf.cat([c[:, j0:j1], mpf.zeros((1, J, Nc), L)], 0)                    # (d | 0, J, Nc)
-            cb = mpf.from_pairs(cbv, L, (R1 * R1, 1, 1))
             cbx = mpf.MPF(cb.m.expand(L, R1 * R1, J, Nc).contiguous(), cb.e.expand(R1 * R1, J, Nc).contiguous())
             P = mpf.add(mpf.mul(cz.take(cidx), cbx), Hz.take(hidx))                     # ((t, e), J, Nc)
             P = P.reshape(R1, R1, J, Nc).permute(2, 0, 1, 3).reshape(J, R1, R1 * Nc)    # (J, t, e Nc)
@@ -312,9 +306,26 @@ class Derivs:
             outs.append(num.reshape(J, self.Dout, R1, 2, S2, B))
         return outs[0] if len(outs) == 1 else mpf.cat(outs, 0)
 
+    def _finish_consts(self):
+        """index tables and binomials of finish() for this R (device constants, built once so that finish()
+        itself only launches device work and can be captured in a graph)"""
+        R1, Rw, L, dev = self.R + 1, max(self.R, 1), self.L, mpf.dev()
+        jidx = torch.tensor([(tj - self.plan.tj12 % 2) // 2 for tj in self.tjs], device=dev)
+        te = [(t, e) for t in range(R1) for e in range(R1)]
+        cidx = torch.tensor([e - t if t <= e else R1 for t, e in te], device=dev)
+        hidx = torch.tensor([e * R1 + (e - t) if t <= e else R1 * R1 for t, e in te], device=dev)
+        cbv = [mpf.norm_int(comb(e, t), 0, L) if t <= e else (0, mpf.EZ) for t, e in te]
+        gidx = torch.tensor([(s - 1 - d) * R1 + d if 1 <= s and d <= s - 1 else Rw * R1
+                             for s in range(R1) for d in range(R1)], device=dev)
+        cb = mpf.from_pairs(cbv, L, (R1 * R1, 1, 1))
+        self._fc = (jidx, cidx, hidx, gidx, cb)
+
     def for_order(self, R):
-        """the same stages for a smaller R (a higher lambda order of the task), sharing the constants"""
+        """the same stages for a smaller R (a higher lambda order of the task): the constants are built once
+        per R and cached; each call returns its own object for the run state (start / accumulate / finish)"""
         assert R <= self.R
+        if R in self._orders:
+            return copy.copy(self._orders[R])       # shared constants, own run state (start/accumulate)
         d = object.__new__(Derivs)
         d.__dict__.update(self.__dict__)
         d.R = R
@@ -332,4 +343,6 @@ class Derivs:
         d.NK = torch.from_numpy(np.array([[(x >> (32 * l)) & 0xFFFFFFFF for x in v] for l in range(d.KL)],
                                          dtype=np.uint32).view(np.int32)).to(mpf.dev())
         d.NKb = torch.tensor([x.bit_length() for x in v], dtype=torch.int32, device=mpf.dev())
-        return d
+        d._finish_consts()
+        self._orders[R] = d
+        return copy.copy(d)

@@ -152,22 +152,24 @@ class Output:
             W = mpf.from_mp([x for row in Wl for x in row], L, (len(outputs), len(inputs)))
         self.W, self.inputs, self.outputs = W, list(inputs), list(outputs)
         self.nL = (Lam - parity) // 2 + 1
+        # rows of each lambda order's derivatives that the map reads (device index tables, built once)
+        self._rows = {}
+        for idx in sorted({i for i, _ in self.inputs}):
+            self._rows[idx] = torch.tensor([e for i, e in self.inputs if i == idx], device=mpf.dev())
 
     def apply_per_order(self, nums):
         """nums[idx]: MPF (Jo, Dout_idx, R_idx + 1, 2, S2, nq) for the task's lambda orders idx (all of them for
         xt, the first for xt_radial).  Returns MPF (Jo, 2, S2, nq, outputs, Dmax)."""
         Dmax = max(x.shape[1] for x in nums)
         rows = []
-        dev = mpf.dev()
         for idx, x in enumerate(nums):
-            e_needed = [e for i, e in self.inputs if i == idx]
-            if not e_needed:
+            if idx not in self._rows:
                 continue
             Jo, D, R1, _, S2, nq = x.shape
             X = x.permute(2, 0, 3, 4, 5, 1)                                      # (e, jo, p, s, q, d)
             if D < Dmax:
                 X = mpf.cat([X, mpf.zeros((R1, Jo, 2, S2, nq, Dmax - D), self.L)], 5)
-            rows.append(X.reshape(R1, -1).take(torch.tensor(e_needed, device=dev)))
+            rows.append(X.reshape(R1, -1).take(self._rows[idx]))
         X = mpf.cat(rows, 0)
         Jo, _, _, _, S2, nq = nums[0].shape
         res = rns.gemm(self.W, X).reshape(len(self.outputs), Jo, 2, S2, nq, Dmax)

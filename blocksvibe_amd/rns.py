@@ -9,7 +9,10 @@ Tables: the nmod largest primes below 2^15 with product M > 2^(2 NB + ceil(log2 
 the byte values 2^(8 j) mod p for the MFMA residue conversion; the CRT constants (M/p)^-1 mod p and the
 signed radix-256 digits of M/p and M for the MFMA CRT lift.
 """
+import json
 import math
+import os
+import threading
 
 import numpy as np
 import torch
@@ -19,6 +22,80 @@ from . import _lib, mpf
 _PRIMES = None
 _TABLES = {}
 WORKSPACE_BYTES = 1 << 31
+
+# MFMA block tiles of b3d_rns_gemm (16 wm rows x 64 / wm columns).  With BLOCKSVIBE_RNS_TUNE=on the first GEMM
+# of a shape class times every candidate (they compute the same bits) and keeps the fastest for the rest of the
+# process; BLOCKSVIBE_RNS_TUNE=<file> also keeps the choices in that JSON file across processes (and only
+# tunes classes it does not hold).  Unset: the tiles of the file BLOCKSVIBE_RNS_TILES names if any, else wm = 1.
+TILES = (1, 2, 4)
+_TUNED = {}
+_TUNE_LOCK = threading.Lock()
+_TUNE_LOADED = [False]
+
+
+def _tune_mode():
+    return os.environ.get("BLOCKSVIBE_RNS_TUNE", "")
+
+
+def _tune_file():
+    f = _tune_mode()
+    return f if f and f not in ("on", "off") else (os.environ.get("BLOCKSVIBE_RNS_TILES") or None)
+
+
+def _shape_class(L, nb, M, N, K, shared):
+    up = lambda x: 1 << max(0, (int(x) - 1).bit_length())
+    return (L, up(nb), up(M), up(N), up(K), int(bool(shared)))
+
+
+def tuned():
+    """the tile chosen per shape class so far"""
+    with _TUNE_LOCK:
+        return dict(_TUNED)
+
+
+def _load_tuning():
+    f = _tune_file()
+    if _TUNE_LOADED[0] or not f:
+        return
+    _TUNE_LOADED[0] = True
+    try:
+        for k, v in json.load(open(f)).items():
+            _TUNED[tuple(int(x) for x in k.split(","))] = int(v)
+    except (OSError, ValueError):
+        pass
+
+
+def _save_tuning():
+    f = _tune_file()
+    if not f or _tune_mode() in ("", "off"):
+        return
+    tmp = f + ".tmp"
+    with open(tmp, "w") as fh:
+        json.dump({",".join(str(x) for x in k): v for k, v in _TUNED.items()}, fh)
+    os.replace(tmp, f)
+
+
+def _run_tuned(key, run):
+    """run(wm) with the tuned tile of this shape class, tuning it first if it is new (not while a graph is being
+    captured: the default tile is used then)"""
+    with _TUNE_LOCK:
+        _load_tuning()
+        wm = _TUNED.get(key)
+    if wm is not None or _tune_mode() in ("", "off") or torch.cuda.is_current_stream_capturing():
+        run(wm or 1)
+        return
+    times = {}
+    run(TILES[0])                                  # warm-up (code objects, caches); its result is the same
+    for t in TILES:
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        run(t)
+        e1.record()
+        e1.synchronize()
+        times[t] = e0.elapsed_time(e1)
+    with _TUNE_LOCK:
+        _TUNED[key] = min(times, key=times.get)
+        _save_tuning()
 
 
 def primes_desc():
@@ -134,8 +211,9 @@ def _chunk_cols(nmod, nbA, nb, M, N, K, budget):
     return min(nc, N)
 
 
-def gemm_strided(L, nb, M, N, K, A, a_strides, a_shared, B, b_strides, C, c_strides, budget=None):
-    """low-level call: A, B, C are MPF (any shapes) addressed through (batch, row, col) strides"""
+def gemm_strided(L, nb, M, N, K, A, a_strides, a_shared, B, b_strides, C, c_strides, budget=None, tile=None):
+    """low-level call: A, B, C are MPF (any shapes) addressed through (batch, row, col) strides; tile forces an
+    MFMA block tile (else the tuned one)"""
     if nb == 0 or M == 0 or N == 0:
         return C
     nmod = n_moduli(L, K)
@@ -147,16 +225,23 @@ def gemm_strided(L, nb, M, N, K, A, a_strides, a_shared, B, b_strides, C, c_stri
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=A.e.device)
     EB = torch.empty(nb * N, dtype=torch.int64, device=A.e.device)
     P = lambda x: x.data_ptr()
-    _lib.call("b3d_rns_gemm", L, nmod, P(t["primes"]), P(t["w8"]), P(t["wcorr"]), P(t["inv"]), P(t["rp"]),
-              P(t["c8"]), t["CK"],
-              nb, M, N, K, 1 if a_shared else 0,
-              P(A.m), P(A.e), A.m[0].numel(), *a_strides,
-              P(B.m), P(B.e), B.m[0].numel(), *b_strides,
-              P(C.m), P(C.e), C.m[0].numel(), *c_strides, Nc, P(EB), P(ws), ws_bytes, mpf.stream_ptr())
+
+    def run(wm):
+        _lib.call("b3d_rns_gemm", L, nmod, P(t["primes"]), P(t["w8"]), P(t["wcorr"]), P(t["inv"]), P(t["rp"]),
+                  P(t["c8"]), t["CK"],
+                  nb, M, N, K, 1 if a_shared else 0,
+                  P(A.m), P(A.e), A.m[0].numel(), *a_strides,
+                  P(B.m), P(B.e), B.m[0].numel(), *b_strides,
+                  P(C.m), P(C.e), C.m[0].numel(), *c_strides, Nc, wm, P(EB), P(ws), ws_bytes, mpf.stream_ptr())
+
+    if tile is not None:
+        run(tile)
+    else:
+        _run_tuned(_shape_class(L, nb, M, N, K, a_shared), run)
     return C
 
 
-def gemm(A, B, out=None, budget=None):
+def gemm(A, B, out=None, budget=None, tile=None):
     """A: MPF (M, K) (shared) or (nb, M, K); B: MPF (K, N) or (nb, K, N) -> MPF (M, N) or (nb, M, N)"""
     A, B = A.contiguous(), B.contiguous()
     L = A.L
@@ -167,7 +252,7 @@ def gemm(A, B, out=None, budget=None):
         K2, N = B.shape
         assert K == K2
         C = out or mpf.zeros((Mr, N), L, A.e.device)
-        return gemm_strided(L, 1, Mr, N, K, A, (0, K, 1), True, B, (0, N, 1), C, (0, N, 1), budget)
+        return gemm_strided(L, 1, Mr, N, K, A, (0, K, 1), True, B, (0, N, 1), C, (0, N, 1), budget, tile)
     nb, K, N = B.shape
     if len(A.shape) == 2:
         Mr, K2 = A.shape
@@ -178,4 +263,4 @@ def gemm(A, B, out=None, budget=None):
         shared, asb = False, Mr * K2
     assert K == K2, (A.shape, B.shape)
     C = out or mpf.zeros((nb, Mr, N), L, A.e.device)
-    return gemm_strided(L, nb, Mr, N, K, A, (asb, K, 1), shared, B, (K * N, N, 1), C, (Mr * N, N, 1), budget)
+    return gemm_strided(L, nb, Mr, N, K, A, (asb, K, 1), shared, B, (K * N, N, 1), C, (Mr * N, N, 1), budget, tile)

@@ -7,7 +7,9 @@ run them as one batch: one recursion per lambda order over all their structures,
 numerator GEMMs and the write_output GEMM.  Requests that arrive while batches are on the GPU join the
 next batch as soon as a worker and memory are free, and batches on different streams overlap on the
 device.  Plans, uploaded level tables, operator constants (M, qL, R tables) and write_output maps are cached
-on the GPU across requests.
+on the GPU across requests.  A batch shape (operator, sign, batch size, coordinates) that has run once is
+captured as a HIP graph the next time and replayed from then on (static inputs refilled by device copies),
+within a memory budget for the graphs' private pools.
 
 Every kernel of the chain treats each right-hand-side column independently (per-column alignment, one
 rounding per entry, A operands that do not depend on the batch), so a request's result is bit-identical
@@ -84,8 +86,30 @@ class _Operator:
         return pl.Smax ** 2 * (4 * L + 8) * 2 * (pl.order + 1) * pl.Jh
 
 
+class _Graph:
+    """one captured request pipeline (operator, sign, batch size, coordinates): static h_inf inputs, static
+    outputs; replays are serialised by a lock, and results are copied out before the next replay"""
+
+    def __init__(self, fn, hs):
+        self.hin = [mpf.MPF(h.m.clone(), h.e.clone()) for h in hs]
+        torch.cuda.current_stream().synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        before = torch.cuda.memory_allocated()
+        with torch.cuda.graph(self.graph, capture_error_mode="thread_local"):
+            self.out = fn(self.hin)
+        self.bytes = max(0, torch.cuda.memory_allocated() - before)
+        self.lock = threading.Lock()
+
+    def replay(self, hs):
+        for a, b in zip(self.hin, hs):
+            a.m.copy_(b.m)
+            a.e.copy_(b.e)
+        self.graph.replay()
+        return self.out
+
+
 class Server:
-    def __init__(self, hbm_budget=None, streams=2, max_batch=64):
+    def __init__(self, hbm_budget=None, streams=2, max_batch=64, graphs=True, graph_budget=None):
         if hbm_budget is None:
             free, _ = torch.cuda.mem_get_info()
             hbm_budget = int(0.5 * free)
@@ -99,6 +123,12 @@ class Server:
         self.stop = False
         self.completed = []
         self.batch_sizes = []
+        # captured pipelines: a batch shape seen once runs eagerly (building every constant it needs), the
+        # next time it is captured and from then on replayed
+        self.use_graphs = graphs
+        self.graph_budget = graph_budget if graph_budget is not None else self.budget // 4
+        self.graphs, self.seen, self.graph_bytes, self.replays = {}, set(), 0, 0
+        self.graph_lock = threading.Lock()
         self.streams = [torch.cuda.Stream() for _ in range(streams)]
         self.workers = [threading.Thread(target=self._work, args=(s,), daemon=True) for s in self.streams]
         for w in self.workers:
@@ -195,28 +225,61 @@ class Server:
             for r in batch:
                 r.done.set()
 
+    def _pipeline(self, op, key, sign, coords, hs):
+        """device work of one batch: per lambda order a recursion over all right-hand sides with the moment
+        hook, the numerators, then the write_output map per (coordinate, Delta_1+2)"""
+        tj12, tj43, tjmax, order, Lam, kept, L, pp, op_id = key
+        p0 = (1 - sign) // 2
+        der = op.derivs_for(p0)
+        B = hs[0].shape[-1]
+        nums = []
+        for i, h in enumerate(hs):
+            dn = der.for_order(Lam - (p0 + 2 * i))
+            dn.start(B)
+            op.rec.run(h, hook=dn.accumulate)
+            nums.append(dn.finish(h))
+        return {(c, d): self._output(c, Lam, p0, d, L).apply_per_order(nums) for c, d in coords}
+
     def _run(self, batch):
         t0 = time.time()
         head = batch[0]
         op = self._operator(head.key)
-        tj12, tj43, tjmax, order, Lam, kept, L, pp, op_id = head.key
-        p0 = (1 - head.sign) // 2
-        der = op.derivs_for(p0)
         for r in batch:
             r.t_start = t0
-        nums = []
-        for i in range(len(head.hs)):
-            n = p0 + 2 * i
-            h = mpf.cat([r.hs[i] for r in batch], 4) if len(batch) > 1 else head.hs[i]
-            dn = der.for_order(Lam - n)
-            dn.start(len(batch))
-            op.rec.run(h, hook=dn.accumulate)
-            nums.append(dn.finish(h))
-        coords = sorted({(c, r.d12) for r in batch for c in r.coords})
-        outs = {}
-        for c, d in coords:
-            o = self._output(c, Lam, p0, d, L)
-            outs[(c, d)] = o.apply_per_order(nums)              # (Jo, 2, S2, nq, outputs, Dmax)
+        coords = tuple(sorted({(c, r.d12) for r in batch for c in r.coords}))
+        nh = len(head.hs)
+        hs = [mpf.cat([r.hs[i] for r in batch], 4) if len(batch) > 1 else head.hs[i] for i in range(nh)]
+        gkey = (head.key, head.sign, len(batch), nh, coords)
+        fn = lambda x: self._pipeline(op, head.key, head.sign, coords, x)
+        graph, capture = None, False
+        if self.use_graphs:
+            with self.graph_lock:
+                graph = self.graphs.get(gkey)
+                capture = gkey not in self.graphs and gkey in self.seen and self.graph_bytes < self.graph_budget
+                if capture:
+                    self.graphs[gkey] = None            # claimed: other workers run this shape eagerly meanwhile
+                self.seen.add(gkey)
+        if capture:
+            # no lock is held while capturing: the pipeline takes the setup lock for cached constants
+            graph = _Graph(fn, hs)
+            with self.graph_lock:
+                self.graphs[gkey] = graph
+                self.graph_bytes += graph.bytes
+        if graph is not None:
+            with graph.lock:
+                outs = graph.replay(hs)
+                self.replays += 1
+                self._deliver(batch, outs, copy=True)
+                torch.cuda.current_stream().synchronize()
+        else:
+            self._deliver(batch, fn(hs), copy=False)
+
+    @staticmethod
+    def _deliver(batch, outs, copy):
         for q, r in enumerate(batch):
-            r.result = {k: mpf.MPF(v.m[:, :, :, :, q].contiguous(), v.e[:, :, :, q].contiguous())
-                        for k, v in outs.items() if (k[0] in r.coords and k[1] == r.d12)}
+            res = {}
+            for k, v in outs.items():
+                if k[0] in r.coords and k[1] == r.d12:
+                    m, e = v.m[:, :, :, :, q], v.e[:, :, :, q]
+                    res[k] = mpf.MPF(m.clone() if copy else m.contiguous(), e.clone() if copy else e.contiguous())
+            r.result = res

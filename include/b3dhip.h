@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-#define B3D_ABI_VERSION 3
+#define B3D_ABI_VERSION 4
 
 /* Last error message of the calling thread (empty if none). */
 const char* b3d_last_error(void);
@@ -99,14 +99,15 @@ int64_t b3d_rns_workspace_bytes(int nmod, int64_t nbA, int64_t nb, int64_t M, in
  * inv[t] = (M/p_t)^-1 mod p_t, rp[t] = 1/p_t with M = prod p_t >= 2^(2(32L-2) + log2 K + 4); CRT digits c8
  * (int8, [CN][CK], CN = 4(2L+2)+2 rounded up to 16, CK = 2 nmod + 1 rounded up to 64): column 2t+1 the signed
  * radix-256 digits of M/p_t, column 2t the same one byte up, column 2 nmod those of M.  EB: scratch of nb*N
- * int64.  Columns are processed Nc at a time within `workspace` (b3d_rns_workspace_bytes).  Requires
- * K <= 2^16, nb < 65536, nmod <= 255. */
+ * int64.  Columns are processed Nc at a time within `workspace` (b3d_rns_workspace_bytes).  tile_wm (1, 2
+ * or 4): MFMA block tile of 16 tile_wm rows x 64 / tile_wm columns (a tuning choice; results do not depend
+ * on it).  Requires K <= 2^16, nb < 65536, nmod <= 255. */
 int b3d_rns_gemm(int L, int nmod, const uint16_t* primes, const int8_t* w8, const int32_t* wcorr, const uint16_t* inv,
                  const double* rp, const int8_t* c8, int CK, int64_t nb, int64_t M, int64_t N, int64_t K, int a_shared,
                  const int32_t* Am, const int64_t* Ae, int64_t aps, int64_t asb, int64_t asi, int64_t ask,
                  const int32_t* Bm, const int64_t* Be, int64_t bps, int64_t bsb, int64_t bsk, int64_t bsj, int32_t* Cm,
-                 int64_t* Ce, int64_t cps, int64_t csb, int64_t csi, int64_t csj, int64_t Nc, int64_t* EB,
-                 void* workspace, int64_t ws_bytes, void* stream);
+                 int64_t* Ce, int64_t cps, int64_t csb, int64_t csi, int64_t csj, int64_t Nc, int tile_wm,
+                 int64_t* EB, void* workspace, int64_t ws_bytes, void* stream);
 
 #ifdef __cplusplus
 }

@@ -50,9 +50,28 @@ def test_rns_gemm_bit_exact(L, nb, M, K, N, shared, budget):
     A = mpf.from_pairs(flatA, L, (M, K) if shared else (nb, M, K))
     B = mpf.from_pairs(flatB, L, (nb, K, N))
     print(f"L={L} nb={nb} M={M} K={K} N={N} shared={shared} nmod={rns.n_moduli(L, K)}", flush=True)
-    C = rns.gemm(A, B, budget=budget).to_host()
+    C = rns.gemm(A, B, budget=budget).to_host()                 # tuned tile (every candidate ran once)
     for b in range(nb):
         ref = R.gemm(As[0] if shared else As[b], Bs[b], L)
         for i in range(M):
             for j in range(N):
                 assert C[(b * M + i) * N + j] == ref[i][j], (b, i, j)
+    for tile in rns.TILES:                                      # every MFMA block tile gives the same bits
+        assert rns.gemm(A, B, budget=budget, tile=tile).to_host() == C, tile
+
+
+def test_tile_tuner(monkeypatch, tmp_path):
+    """BLOCKSVIBE_RNS_TUNE=<file>: the first GEMM of a shape class times every tile, keeps the fastest, writes it to
+    the file; the result is the same bits as the default tile"""
+    from blocksvibe_amd import mpf, rns
+    L, nb, M, K, N = 10, 3, 20, 24, 300
+    rng = random.Random(5)
+    A = mpf.from_pairs([x for _ in range(nb) for row in mat(rng, L, M, K, 20) for x in row], L, (nb, M, K))
+    B = mpf.from_pairs([x for _ in range(nb) for row in mat(rng, L, K, N, 20) for x in row], L, (nb, K, N))
+    want = rns.gemm(A, B, tile=1).to_host()
+    f = tmp_path / "tiles.json"
+    monkeypatch.setenv("BLOCKSVIBE_RNS_TUNE", str(f))
+    got = rns.gemm(A, B).to_host()
+    key = rns._shape_class(L, nb, M, N, K, False)
+    assert got == want and key in rns.tuned() and rns.tuned()[key] in rns.TILES
+    assert f.exists() and ",".join(str(x) for x in key) in f.read_text()
