@@ -26,13 +26,18 @@ static inline unsigned grid_for(int64_t n, int th) {
   return (unsigned)b;
 }
 
-// signed v (|v| < 2^52) mod p in [0, p)
-__device__ __forceinline__ uint32_t mod_signed(int64_t v, uint32_t p, double rp) {
-  const double q = floor((double)v * rp);
-  int64_t r = v - (int64_t)q * (int64_t)p;
-  if (r < 0) r += p;
-  if (r >= (int64_t)p) r -= p;
-  return (uint32_t)r;
+// y = x mod p in [0, p) for an integral double |x| < 2^52 (q within one of x/p, one correction each way)
+__device__ __forceinline__ double fmod_p(double x, double pd, double rp) {
+  double y = fma(-floor(x * rp), pd, x);
+  y = y < 0.0 ? y + pd : y;
+  return y >= pd ? y - pd : y;
+}
+
+// the balanced residue |y| <= p/2 of an integral y in [0, p) as two int8 digits y = 256 h + l, packed h | l << 8
+__device__ __forceinline__ uint16_t digit_pair(double y, double pd) {
+  const int yb = y > 0.5 * pd ? (int)(y - pd) : (int)y;
+  const int h = (yb + 128) >> 8;
+  return (uint16_t)(((u32)h & 0xFFu) | (((u32)(yb - 256 * h) & 0xFFu) << 8));
 }
 
 // (1) exponent maximum per line: entry (b, line, k) at b*sb + line*sl + k*sk; Eline[b * lines + line]
@@ -167,13 +172,14 @@ __global__ __launch_bounds__(64) void k_to_residues(const int32_t* __restrict__ 
 }
 
 // (3) per-prime products on MFMA: each wave one 16 x 16 output tile; the 4 waves of a block cover
-// (16 wm) x (64 / wm) outputs (wm = 1, 2 or 4, chosen by the caller's tuner), staged in LDS and written by
-// rows.  Residues out: Cr[(t * nb + b) * M * Nc + i * Nc + j] (uint16).
+// (16 wm) x (64 / wm) outputs (wm = 1, 2 or 4, chosen by the caller's tuner).  The epilogue reduces each sum
+// mod p and multiplies by (M/p)^-1 mod p in FP64, so it writes the CRT coefficient y_t directly, balanced,
+// as an int8 digit pair; staged in LDS and written by rows: Cr[(t * nb + b) * M * Nc + i * Nc + j].
 __global__ __launch_bounds__(256) void k_rns_mma(const int8_t* __restrict__ A8, i64 aplane, int a_shared,
                                                  const int8_t* __restrict__ B8, i64 bplane, int Mp, int Np, int Kp,
                                                  int M, int N, const uint16_t* __restrict__ primes,
-                                                 const double* __restrict__ rp, uint16_t* __restrict__ Cr, i64 nb,
-                                                 int wm) {
+                                                 const uint16_t* __restrict__ inv, const double* __restrict__ rp,
+                                                 uint16_t* __restrict__ Cr, i64 nb, int wm) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int wn = 4 / wm, bw = 16 * wn;
   const int tm = blockIdx.y;
@@ -205,12 +211,12 @@ __global__ __launch_bounds__(256) void k_rns_mma(const int8_t* __restrict__ A8, 
     mid = __builtin_amdgcn_mfma_i32_16x16x64_i8(al, bh, mid, 0, 0, 0);
     ll = __builtin_amdgcn_mfma_i32_16x16x64_i8(al, bl, ll, 0, 0, 0);
   }
-  const uint32_t p = primes[tm];
-  const double rpt = rp[tm];
+  const double pd = (double)primes[tm], rpt = rp[tm], ivd = (double)inv[tm];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    const int64_t v = (int64_t)hh[i] * 65536 + (int64_t)mid[i] * 256 + (int64_t)ll[i];
-    st[16 * wi + 4 * g + i][16 * wj + r] = (uint16_t)mod_signed(v, p, rpt);
+    // c = sum mod p (|sum| < 2^31, exact in FP64), then the CRT coefficient y = c (M/p)^-1 mod p (< 2^46)
+    const double c = fmod_p(fma((double)hh[i], 65536.0, fma((double)mid[i], 256.0, (double)ll[i])), pd, rpt);
+    st[16 * wi + 4 * g + i][16 * wj + r] = digit_pair(fmod_p(c * ivd, pd, rpt), pd);
   }
   __syncthreads();
   {
@@ -227,14 +233,16 @@ __global__ __launch_bounds__(256) void k_rns_mma(const int8_t* __restrict__ A8, 
 }
 
 // (4) CRT lift on MFMA and normalisation.  With y_t = c_t (M/p_t)^-1 mod p_t taken balanced (|y_t| < 2^14,
-// split y_t = 256 h_t + l_t into int8 digits) and q = round(sum_t y_t / p_t) (|q| <= nmod / 2 < 128),
+// split y_t = 256 h_t + l_t into int8 digits by the MFMA kernel) and q = round(sum_t y_t / p_t)
+// (|q| <= nmod / 2 < 128),
 //   C = sum_t y_t M/p_t - q M
 // is one int8 GEMM against the constant signed radix-256 digits of M/p_t (row l_t), of 256 M/p_t (row h_t)
 // and of M (row -q): c8[byte column][k], k < CK.  Byte columns run along the MFMA rows, so each lane holds
 // the sums of four consecutive byte columns = one 32-bit limb's worth (|sum| < 2^48 in int64).  One wave
 // lifts 64 outputs (4 MFMA column tiles of 16) per pass:
-//   phase 1  lane = output: residues (one coalesced row per prime) -> y digits and -q into an LDS row,
-//            then each lane loads its MFMA operand fragments (output r of tile ot, bytes 64 ks + 16 g ..)
+//   phase 1  lane = output: the y digit pairs (one coalesced row per prime, written by the MFMA kernel)
+//            and -q into an LDS row, then each lane loads its MFMA operand fragments (output r of tile ot,
+//            bytes 64 ks + 16 g ..)
 //   phase 2  per 16 byte columns: CK/64 steps of v_mfma_i32_16x16x64_i8 per tile -> limb sums into LDS
 //   phase 3  lane = output: carry propagation into 2L + 2 limbs, one rounding, store.
 // Output entry (b, i, j) at b*csb + i*csi + j*csj (+ column offset j0).
@@ -245,7 +253,6 @@ constexpr int crt_ckmax() {
 
 template <int L>
 __global__ __launch_bounds__(64) void k_crt(const uint16_t* __restrict__ Cr, i64 nb, int M, int N, int nmod,
-                                            const uint16_t* __restrict__ primes, const uint16_t* __restrict__ inv,
                                             const double* __restrict__ rp, const int8_t* __restrict__ c8, int CK,
                                             const i64* __restrict__ EA, int a_shared, const i64* __restrict__ EB,
                                             i64 j0, i64 Nfull, int32_t* __restrict__ cm, i64* __restrict__ ce,
@@ -278,17 +285,10 @@ __global__ __launch_bounds__(64) void k_crt(const uint16_t* __restrict__ Cr, i64
           for (int u = 0; u < 16; ++u) {
             const int t = t0 + u;
             if (t < nmod) {
-              const uint32_t p = primes[t];
-              const double pd = (double)p;
-              const double v = (double)(c[u] * (uint32_t)inv[t]);   // < 2^30
-              double y = fma(-floor(v * rp[t]), pd, v);
-              y = y < 0.0 ? y + pd : y;
-              y = y >= pd ? y - pd : y;
-              int yb = (int)y;
-              if (yb > (int)(p >> 1)) yb -= (int)p;
-              qs += (double)yb * rp[t];
-              const int h = (yb + 128) >> 8;
-              *(uint16_t*)(yrow + 2 * t) = (uint16_t)(((u32)h & 0xFFu) | (((u32)(yb - 256 * h) & 0xFFu) << 8));
+              // digit pair (h, l) of the balanced y_t: y_t = 256 h + l
+              const int yb = 256 * (int)(int8_t)(c[u] & 0xFFu) + (int)(int8_t)(c[u] >> 8);
+              qs = fma((double)yb, rp[t], qs);
+              *(uint16_t*)(yrow + 2 * t) = (uint16_t)c[u];
             }
           }
         }
@@ -429,9 +429,9 @@ int b3d_rns_gemm(int L, int nmod, const uint16_t* primes, const int8_t* w8, cons
     const int bh = 16 * tile_wm, bw = 64 / tile_wm;
     dim3 grid((unsigned)(((Mpad + bh - 1) / bh) * ((Np + bw - 1) / bw)), (unsigned)nmod, (unsigned)nb);
     k_rns_mma<<<grid, 256, 0, s>>>(A8, aplane, a_shared, B8, nb * (int64_t)Np * Kpad, Mpad, Np, Kpad, (int)M,
-                                   (int)nc, primes, rp, Cr, nb, tile_wm);
+                                   (int)nc, primes, inv, rp, Cr, nb, tile_wm);
     B3D_DISPATCH_L(L, k_crt<LL><<<grid_for(nb * M * nc, 64), 64, 0, s>>>(
-                          Cr, nb, (int)M, (int)nc, nmod, primes, inv, rp, c8, CK, EA, a_shared, EB, j0, N, Cm, Ce,
+                          Cr, nb, (int)M, (int)nc, nmod, rp, c8, CK, EA, a_shared, EB, j0, N, Cm, Ce,
                           cps, csb, csi, csj));
     if (b3d::check_launch("b3d_rns_gemm (chunk)")) return 1;
   }
