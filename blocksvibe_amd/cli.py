@@ -323,6 +323,12 @@ def max_rhs(plan, der, L, free=None):
     per = S2 * nb * ((plan.nphys + 1) + 2 * der.Jo * der.NP * (der.R + 1) + 8 * 2 * der.Jo * der.NP
                      + 2 * (plan.order + 1) * plan.Jh)
     n = max(1, int(0.6 * free / (2 * per)))
+    # every limb plane of the chain stays below 2^30 entries (32-bit plane offsets in the kernels): residue
+    # slots, moments, numerators (Jo, Dout, R+1, 2, S2, B) and the write_output result (Jo, 2, S2, B, outputs,
+    # Dmax), with Dout, outputs <= 4 (R + 1) + NP as generous bounds
+    wide = max(plan.nphys + 1, 2 * der.Jo * der.NP * (der.R + 1),
+               der.Jo * (4 * (der.R + 1) + der.NP) ** 2 * 2) * S2
+    n = max(1, min(n, ((1 << 30) - 1) // wide))
     cap = os.environ.get("BLOCKSVIBE_MAX_RHS")
     return min(n, int(cap)) if cap else n
 

@@ -72,6 +72,7 @@ int b3d_moments_stage(int L, const int32_t* rm, const int64_t* re, int64_t rps, 
                       const int64_t* pe, int64_t pps, int32_t* sm, int64_t* se, int64_t sps, int64_t woff,
                       void* stream) {
   if (T <= 0 || N <= 0) return 0;
+  if (b3d::check_planes("b3d_moments_stage", {rps, pps, sps})) return 2;
   hipStream_t s = (hipStream_t)stream;
   B3D_DISPATCH_L(L, k_mom_stage<LL><<<grid_for(T * N, 256), 256, 0, s>>>(rm, re, rps, slot, row, nn, T, N, pm, pe,
                                                                           pps, sm, se, sps, woff));
@@ -82,6 +83,7 @@ int b3d_moments_flush(int L, int KL, const int32_t* sm, const int64_t* se, int64
                       const int32_t* srow, const uint32_t* NK, int64_t nkps, const int32_t* NKb, int32_t* vm,
                       int64_t* ve, int64_t vps, int64_t nrows, int64_t N, int R1, void* stream) {
   if (nrows <= 0 || N <= 0 || R1 <= 0) return 0;
+  if (b3d::check_planes("b3d_moments_flush", {sps, vps})) return 2;
   hipStream_t s = (hipStream_t)stream;
   const int64_t n = nrows * N * R1;
   if (KL <= 4) {

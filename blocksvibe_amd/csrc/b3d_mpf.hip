@@ -103,7 +103,7 @@ __global__ __launch_bounds__(256) void k_align_rows(const int32_t* __restrict__ 
     i64 E = B3D_EZ;
     for (i64 j = 0; j < K; ++j) {
       const i64 idx = r * rs + j * ks;
-      if (m[(L - 1) * ps + idx] != 0) {
+      if (mp::load_limb(m, ps, L - 1, idx) != 0u) {
         const i64 ej = e[idx];
         E = ej > E ? ej : E;
       }
@@ -148,6 +148,7 @@ int b3d_mpf_norm_wide(int L, int LW, const int32_t* W, int64_t wps, const int64_
     b3d::set_error("b3d_mpf_norm_wide: need 1 <= LW <= 2L+4");
     return 2;
   }
+  if (b3d::check_planes("b3d_mpf_norm_wide", {ops})) return 2;
   hipStream_t s = (hipStream_t)stream;
   B3D_DISPATCH_L(L, k_norm_wide<LL><<<dim3(grid_for(n, 256)), dim3(256), 0, s>>>(W, wps, LW, ein, om, oe, ops, n));
   return b3d::check_launch("b3d_mpf_norm_wide");
@@ -157,6 +158,7 @@ int b3d_mpf_add(int L, const int32_t* am, const int64_t* ae, int64_t aps, int64_
                 const int64_t* be, int64_t bps, int64_t sb, int32_t* cm, int64_t* ce, int64_t cps, int64_t n,
                 void* stream) {
   if (n <= 0) return 0;
+  if (b3d::check_planes("b3d_mpf_add", {aps, bps, cps})) return 2;
   hipStream_t s = (hipStream_t)stream;
   B3D_DISPATCH_L(L, k_add<LL><<<dim3(grid_for(n, 256)), dim3(256), 0, s>>>(am, ae, aps, sa, bm, be, bps, sb, cm, ce, cps, n));
   return b3d::check_launch("b3d_mpf_add");
@@ -166,6 +168,7 @@ int b3d_mpf_mul(int L, const int32_t* am, const int64_t* ae, int64_t aps, int64_
                 const int64_t* be, int64_t bps, int64_t sb, int32_t* cm, int64_t* ce, int64_t cps, int64_t n,
                 void* stream) {
   if (n <= 0) return 0;
+  if (b3d::check_planes("b3d_mpf_mul", {aps, bps, cps})) return 2;
   hipStream_t s = (hipStream_t)stream;
   B3D_DISPATCH_L(L, k_mul<LL><<<dim3(grid_for(n, 256)), dim3(256), 0, s>>>(am, ae, aps, sa, bm, be, bps, sb, cm, ce, cps, n));
   return b3d::check_launch("b3d_mpf_mul");
@@ -174,6 +177,7 @@ int b3d_mpf_mul(int L, const int32_t* am, const int64_t* ae, int64_t aps, int64_
 int b3d_mpf_align_rows(int L, const int32_t* m, const int64_t* e, int64_t ps, int64_t rows, int64_t K,
                        int64_t rs, int64_t ks, int32_t* om, int64_t ops, int64_t* oe, void* stream) {
   if (rows <= 0 || K <= 0) return 0;
+  if (b3d::check_planes("b3d_mpf_align_rows", {ps, ops})) return 2;
   hipStream_t s = (hipStream_t)stream;
   B3D_DISPATCH_L(L, k_align_rows<LL><<<dim3(grid_for(rows * K, 256)), dim3(256), 0, s>>>(m, e, ps, rows, K, rs, ks, om, ops, oe));
   return b3d::check_launch("b3d_mpf_align_rows");

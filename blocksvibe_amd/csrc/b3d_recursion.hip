@@ -92,6 +92,7 @@ int b3d_cauchy_level(int L, const int32_t* rm, const int64_t* re, int64_t rps, i
                      int64_t nH, const int64_t* col, const int32_t* cold, const int32_t* ncol, int Kc, int32_t* om,
                      int64_t* oe, int64_t ops, void* stream) {
   if (nH <= 0 || N <= 0) return 0;
+  if (b3d::check_planes("b3d_cauchy_level", {rps, hps, ops})) return 2;
   hipStream_t s = (hipStream_t)stream;
   B3D_DISPATCH_L(L, k_cauchy<LL><<<dim3(grid_for(nH * N, 256)), dim3(256), 0, s>>>(rm, re, rps, N, hm, he, hps, ghrow, rg, rd, nH, col, cold, ncol, Kc, om, oe, ops));
   return b3d::check_launch("b3d_cauchy_level");
@@ -107,6 +108,7 @@ int b3d_pole_level(int L, int S, int64_t B, int64_t T, const int32_t* Hm, const 
     b3d::set_error("b3d_pole_level: structure count S must be 1..16");
     return 2;
   }
+  if (b3d::check_planes("b3d_pole_level", {hps, rtps, qps, yps, rps})) return 2;
   hipStream_t s = (hipStream_t)stream;
   const int64_t n = T * S * S * B;
   B3D_DISPATCH_L(L, k_pole_r<LL><<<dim3(grid_for(n, 256)), dim3(256), 0, s>>>(Hm, He, hps, thrw, tR, T, S, B, Rm, Re, rtps, Ym, Ye, yps));

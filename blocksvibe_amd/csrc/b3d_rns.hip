@@ -116,6 +116,11 @@ __global__ __launch_bounds__(64) void k_to_residues(const int32_t* __restrict__ 
     for (int et = 0; et < 4; ++et)
 #pragma unroll
       for (int ks = 0; ks < NKS; ++ks) A[et][ks] = *(const v4i*)(xb + (et * 16 + r) * ROW + ks * 64 + g * 16);
+    u32 negs = 0;   // sign flags of this lane's 16 output entries et*16 + 4g + i, bit et*4 + i
+#pragma unroll
+    for (int et = 0; et < 4; ++et)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) negs |= (sgn[et * 16 + 4 * g + i] ? 1u : 0u) << (et * 4 + i);
     for (int pt = 0; pt < np16; pt += 16) {
       // phase 2
       v4i Bh[NKS], Bl[NKS];
@@ -126,9 +131,9 @@ __global__ __launch_bounds__(64) void k_to_residues(const int32_t* __restrict__ 
       }
       const int tm = pt + r;
       const bool live = tm < nmod;
-      const uint32_t p = live ? primes[tm] : 1u;
+      const double pd = live ? (double)primes[tm] : 1.0;
       const double rpt = live ? rp[tm] : 1.0;
-      const i64 corr = live ? wcorr[tm] : 0;
+      const double corrd = live ? (double)wcorr[tm] : 0.0;
 #pragma unroll
       for (int et = 0; et < 4; ++et) {
         v4i ah = {0, 0, 0, 0}, al = {0, 0, 0, 0};
@@ -137,22 +142,15 @@ __global__ __launch_bounds__(64) void k_to_residues(const int32_t* __restrict__ 
           ah = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[et][ks], Bh[ks], ah, 0, 0, 0);
           al = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[et][ks], Bl[ks], al, 0, 0, 0);
         }
-        // phase 3: entries et*16 + 4g + i, prime tm
+        // phase 3: entries et*16 + 4g + i, prime tm, all in FP64 (|256 Yh + Yl + wcorr| < 2^31: exact)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int ent = et * 16 + 4 * g + i;
-          const double y = (double)((i64)ah[i] * 256 + (i64)al[i] + corr);   // |y| < 2^31: exact
-          const double q = floor(y * rpt);
-          double rr = fma(-q, (double)p, y);
-          rr = rr < 0.0 ? rr + (double)p : rr;
-          rr = rr >= (double)p ? rr - (double)p : rr;
-          int rb = (int)rr;
-          if (sgn[ent] && rb) rb = (int)p - rb;
-          if (rb > (int)(p >> 1)) rb -= (int)p;  // balanced, |rb| < 2^14
-          const int h = (rb + 128) >> 8;
-          const int l = rb - 256 * h;
-          stage[(r * 2 + 0) * 64 + ent] = (int8_t)h;
-          stage[(r * 2 + 1) * 64 + ent] = (int8_t)l;
+          double c = fmod_p(fma((double)ah[i], 256.0, (double)al[i] + corrd), pd, rpt);
+          c = ((negs >> (et * 4 + i)) & 1u) && c != 0.0 ? pd - c : c;    // residue of the signed entry
+          const uint16_t hl = digit_pair(c, pd);
+          stage[(r * 2 + 0) * 64 + ent] = (int8_t)(hl & 0xFFu);
+          stage[(r * 2 + 1) * 64 + ent] = (int8_t)(hl >> 8);
         }
       }
       __syncthreads();
@@ -399,6 +397,7 @@ int b3d_rns_gemm(int L, int nmod, const uint16_t* primes, const int8_t* w8, cons
     b3d::set_error("b3d_rns_gemm: workspace too small for the column chunk");
     return 2;
   }
+  if (b3d::check_planes("b3d_rns_gemm", {aps, bps, cps})) return 2;
   hipStream_t s = (hipStream_t)stream;
   const int Mpad = (int)((M + 15) / 16 * 16), Npad = (int)((Nc + 63) / 64 * 64), Kpad = (int)((K + 15) / 16 * 16);
   char* w = (char*)workspace;

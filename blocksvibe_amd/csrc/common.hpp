@@ -4,9 +4,24 @@
 #include <stdint.h>
 #include <string>
 
+#include <initializer_list>
+
 namespace b3d {
 void set_error(const std::string& msg);
 int check_launch(const char* what);
+
+// Limb planes are addressed with 32-bit byte offsets (buffer loads, see mp.hpp): every plane stride (entries
+// per limb plane, which bounds every index into it) must stay below 2^30.  Returns 0, or sets the error and
+// returns 2.
+inline int check_planes(const char* what, std::initializer_list<int64_t> strides) {
+  for (int64_t ps : strides)
+    if (ps < 0 || ps >= (int64_t(1) << 30)) {
+      set_error(std::string(what) + ": a limb plane of " + std::to_string(ps) +
+                " entries exceeds the 2^30 entries that 32-bit plane offsets address (split the batch)");
+      return 2;
+    }
+  return 0;
+}
 }  // namespace b3d
 
 // Limb counts with compiled kernels.  L = 2 ceil(precision / 64) + 2 (precision rounded up to whole

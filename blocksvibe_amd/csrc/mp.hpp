@@ -255,18 +255,61 @@ __device__ __forceinline__ i64 norm(u32 (&acc)[NW], i64 e, u32 (&m)[L]) {
   return e + s;
 }
 
-// load / store an entry of a limb-plane array (plane stride ps)
+// load / store an entry of a limb-plane array (plane stride ps entries, base p uniform over the wave).
+// On the GPU each limb is one buffer instruction: the plane base walks in SGPRs (the empty asm keeps the
+// compiler from hoisting all N plane bases into scalar registers, which would spill) and the per-lane
+// offset idx * 4 is a 32-bit VGPR, so no VALU address arithmetic is spent per limb.  The entry points check
+// that every plane stride is below 2^30 entries.
+#ifdef B3D_HOST_CHECK
 template <int N>
 __device__ __forceinline__ void load(u32 (&a)[N], const int32_t* __restrict__ p, i64 ps, i64 idx) {
-#pragma unroll
   for (int k = 0; k < N; ++k) a[k] = (u32)p[k * ps + idx];
 }
 
 template <int N>
 __device__ __forceinline__ void store(const u32 (&a)[N], int32_t* __restrict__ p, i64 ps, i64 idx) {
-#pragma unroll
   for (int k = 0; k < N; ++k) p[k * ps + idx] = (int32_t)a[k];
 }
+
+__device__ __forceinline__ u32 load_limb(const int32_t* __restrict__ p, i64 ps, int k, i64 idx) {
+  return (u32)p[k * ps + idx];
+}
+#else
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const char* base) {
+  // raw buffer (stride 0), every 32-bit offset in range, 32-bit data format (dword 3 = 0x00020000)
+  return __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, (int)0xFFFFFFFF, 0x00020000);
+}
+
+template <int N>
+__device__ __forceinline__ void load(u32 (&a)[N], const int32_t* __restrict__ p, i64 ps, i64 idx) {
+  const u32 off = (u32)idx * 4u;
+  const char* base = (const char*)p;
+  const i64 step = ps * 4;
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    a[k] = __builtin_amdgcn_raw_buffer_load_b32(plane_rsrc(base), off, 0, 0);
+    base += step;
+    asm volatile("" : "+s"(base));
+  }
+}
+
+template <int N>
+__device__ __forceinline__ void store(const u32 (&a)[N], int32_t* __restrict__ p, i64 ps, i64 idx) {
+  const u32 off = (u32)idx * 4u;
+  const char* base = (const char*)p;
+  const i64 step = ps * 4;
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    __builtin_amdgcn_raw_buffer_store_b32(a[k], plane_rsrc(base), off, 0, 0);
+    base += step;
+    asm volatile("" : "+s"(base));
+  }
+}
+
+__device__ __forceinline__ u32 load_limb(const int32_t* __restrict__ p, i64 ps, int k, i64 idx) {
+  return __builtin_amdgcn_raw_buffer_load_b32(plane_rsrc((const char*)p + k * ps * 4), (u32)idx * 4u, 0, 0);
+}
+#endif
 
 // a + b for normalised operands: both put on the grid 2^(max(ea,eb) - 32), the smaller truncated, exact
 // sum in L + 3 limbs, one rounding.  a, b are two's complement (not modified); result in r.
@@ -410,7 +453,7 @@ __device__ __forceinline__ i64 small_dot(const int32_t* __restrict__ am, i64 aps
   i64 E = B3D_EZ;
   for (int j = 0; j < S; ++j) {
     const i64 idx = xbase + j * xstride;
-    if (xm[(L - 1) * xps + idx] != 0) {
+    if (load_limb(xm, xps, L - 1, idx) != 0u) {
       const i64 ej = xe[idx];
       E = ej > E ? ej : E;
     }
@@ -475,7 +518,7 @@ __device__ __forceinline__ i64 cauchy_entry(const int32_t* __restrict__ rm, cons
   i64 E = B3D_EZ;
   for (int i = 0; i < nc; ++i) {
     const i64 idx = colg[i] * N + n;
-    if (rm[(L - 1) * rps + idx] != 0) {
+    if (load_limb(rm, rps, L - 1, idx) != 0u) {
       const i64 ei = re[idx];
       E = ei > E ? ei : E;
     }
@@ -537,7 +580,7 @@ __device__ __forceinline__ i64 mom_window(const int32_t* __restrict__ sm, const 
   i64 E = B3D_EZ;
   for (int w = 0; w < W; ++w) {
     const i64 o = ((i64)w * nrows + row) * N + c;
-    if (sm[(L - 1) * sps + o] == 0) continue;
+    if (load_limb(sm, sps, L - 1, o) == 0u) continue;
     const i64 x = se[o] + NBITS + NKb[(i64)(n0 + w) * R1 + k];
     E = x > E ? x : E;
   }
