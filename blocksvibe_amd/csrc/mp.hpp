@@ -112,11 +112,12 @@ __device__ __forceinline__ int bitlen(const u32 (&a)[N]) {
   return bl;
 }
 
-// out[0..NO) = in >> s (logical, NI input limbs, s >= 0).  Barrel stages from the largest: after the stage
-// of 2^b limbs, only the NO + 2^b lowest limbs can still reach the output.  Stages no lane needs are
-// skipped; the bit part is one funnel shift per output limb.
+// out[0..NO) = in >> s (NI input limbs, s >= 0), the vacated bits filled from `fill` (0: logical shift;
+// the sign mask of a two's complement input: arithmetic shift = floor division by 2^s).  Barrel stages from
+// the largest: after the stage of 2^b limbs, only the NO + 2^b lowest limbs can still reach the output.
+// Stages no lane needs are skipped; the bit part is one funnel shift per output limb.
 template <int NO, int NI>
-__device__ __forceinline__ void shr_to(u32 (&out)[NO], u32 (&a)[NI], int s) {
+__device__ __forceinline__ void shr_to(u32 (&out)[NO], u32 (&a)[NI], int s, u32 fill = 0u) {
   int q = s >> 5;
   if (q > NI) q = NI;
   constexpr int TOP = (NI >= 64) ? 6 : (NI >= 32) ? 5 : (NI >= 16) ? 4 : (NI >= 8) ? 3 : (NI >= 4) ? 2 : (NI >= 2) ? 1 : 0;
@@ -127,15 +128,15 @@ __device__ __forceinline__ void shr_to(u32 (&out)[NO], u32 (&a)[NI], int s) {
     if (B3D_ANY(on)) {
 #pragma unroll
       for (int k = 0; k < NI; ++k)
-        if (k < NO + st) a[k] = on ? (k + st < NI ? a[k + st] : 0u) : a[k];
+        if (k < NO + st) a[k] = on ? (k + st < NI ? a[k + st] : fill) : a[k];
     }
   }
-  // q == NI (the clamp) has already moved zeros into every limb the output reads
+  // q == NI (the clamp) has already moved fill limbs into every limb the output reads
   const u32 r = (u32)(s & 31);
 #pragma unroll
   for (int k = 0; k < NO; ++k) {
-    const u32 lo = k < NI ? a[k] : 0u;
-    const u32 hi = k + 1 < NI ? a[k + 1] : 0u;
+    const u32 lo = k < NI ? a[k] : fill;
+    const u32 hi = k + 1 < NI ? a[k + 1] : fill;
     out[k] = b3d_alignbit(hi, lo, r);
   }
 }
@@ -190,12 +191,13 @@ __device__ __forceinline__ void sub_from(u32 (&acc)[N], const u32 (&x)[M]) {
   for (int k = 0; k < N; ++k) acc[k] = subb(acc[k], k < M ? x[k] : 0u, b);
 }
 
-// acc += x if m == 0, acc -= x if m == ~0 (x a zero-extended magnitude of M <= N limbs)
+// acc += x if m == 0, acc -= x if m == ~0 (x of M <= N limbs, extended with `fill`: 0 for a magnitude, the
+// sign mask for a two's complement value)
 template <int N, int M>
-__device__ __forceinline__ void cadd(u32 (&acc)[N], const u32 (&x)[M], u32 m) {
+__device__ __forceinline__ void cadd(u32 (&acc)[N], const u32 (&x)[M], u32 m, u32 fill = 0u) {
   u32 c = m & 1u;
 #pragma unroll
-  for (int k = 0; k < N; ++k) acc[k] = addc(acc[k], (k < M ? x[k] : 0u) ^ m, c);
+  for (int k = 0; k < N; ++k) acc[k] = addc(acc[k], (k < M ? x[k] : fill) ^ m, c);
 }
 
 // p = a * b (magnitudes), p has NA + NBL limbs
@@ -357,22 +359,27 @@ __device__ __forceinline__ i64 add(const u32 (&a)[L], i64 ea, const u32 (&b)[L],
   return norm<L, L + 3>(acc, tt, r);
 }
 
-// q = floor(|x| * 2^32 / d) for a magnitude x of L limbs and 0 < d < 2^15: L + 1 quotient limbs by long
-// division one 32-bit limb at a time in FP64 (full rate on CDNA4).  The partial dividend
-// cur = rem 2^32 + limb < 2^47 is exact; fma(cur, 1/d, 2^-19) is within 2^-20 of cur/d + 2^-19 (rounding
-// of 1/d and of the fma, each < 2^-21 on a value < 2^32), and the fractional part of cur/d is at most
-// 1 - 1/d < 1 - 2^-15, so its floor is the exact digit; the remainder cur - q d is exact.
+// q = floor(m * 2^32 / d) for a signed mantissa m (two's complement, L limbs) and 0 < d < 2^15: L + 1
+// quotient limbs in two's complement (the top digit signed, the others in [0, 2^32)), by long division one
+// 32-bit limb at a time in FP64 (full rate on CDNA4) with remainders in [0, d).  The partial dividend
+// cur = rem 2^32 + limb (|cur| < 2^47; the top one is the signed top limb) is exact; fma(cur, 1/d, 2^-19)
+// is within 2^-20 of cur/d + 2^-19 (rounding of 1/d and of the fma, each < 2^-21 on a value below 2^32 in
+// magnitude), and the fractional part of cur/d is at most 1 - 1/d < 1 - 2^-15, so its floor is the exact
+// digit; the remainder cur - q d is exact.
 template <int L>
 __device__ __forceinline__ void div_small(const u32 (&x)[L], u32 d, u32 (&q)[L + 1]) {
   const double dd = (double)d;
   const double rd = 1.0 / dd;
   const double bias = 1.0 / 524288.0;   // 2^-19
-  double rem = 0.0;
+  double cur = (double)(int32_t)x[L - 1];
+  double qd = floor(fma(cur, rd, bias));
+  double rem = fma(-qd, dd, cur);
+  q[L] = (u32)(int32_t)qd;
 #pragma unroll
-  for (int k = L; k >= 0; --k) {
+  for (int k = L - 1; k >= 0; --k) {
     const u32 limb = k > 0 ? x[k - 1] : 0u;
-    const double cur = fma(rem, 4294967296.0, (double)limb);
-    const double qd = floor(fma(cur, rd, bias));
+    cur = fma(rem, 4294967296.0, (double)limb);
+    qd = floor(fma(cur, rd, bias));
     rem = fma(-qd, dd, cur);
     q[k] = (u32)qd;
   }
@@ -507,8 +514,9 @@ __device__ __forceinline__ i64 mul_norm(u32 (&a)[L], u32 (&b)[L], i64 e, u32 (&o
 // slots colg[i] (i < nc) at doubled positions coldg[i], h the h_inf entry hidx of hm/he, with one rounding.
 // E = largest exponent of the nonzero x_i (slot buffer rm/re, plane stride rps, N columns per slot);
 // grid G = max(E - 31, e_h - 32) (either side alone when the other is absent); term i =
-// sign * floor(|x_i| 2^32 / |d|) >> (G + 31 - e_i) and h truncated onto 2^G; exact sum in L + 3 limbs,
-// one rounding.  Returns the exponent, mantissa in r.
+// sign(d) floor(m_i 2^32 / |d| / 2^(G + 31 - e_i)) (signed mantissa, floor toward -infinity; none when the
+// shift reaches 32 (L + 1)) and h truncated onto 2^G; exact sum in L + 3 limbs, one rounding.  Returns the
+// exponent, mantissa in r.
 template <int L>
 __device__ __forceinline__ i64 cauchy_entry(const int32_t* __restrict__ rm, const i64* __restrict__ re, i64 rps,
                                             i64 N, i64 n, const int64_t* __restrict__ colg,
@@ -544,14 +552,13 @@ __device__ __forceinline__ i64 cauchy_entry(const int32_t* __restrict__ rm, cons
       if (x[L - 1] == 0u) continue;
       const i64 sh = G + 31 - re[idx];
       if (sh >= 32 * (L + 1)) continue;
-      u32 sg = to_mag(x);
       const u32 sd = 0u - (u32)(d < 0);
-      sg ^= sd;
       d = d < 0 ? -d : d;
-      u32 q[L + 1];
-      div_small<L>(x, (u32)d, q);
-      shr(q, (int)sh);
-      cadd(acc, q, sg);
+      u32 q[L + 1], t[L + 1];
+      div_small<L>(x, (u32)d, q);                 // floor(m 2^32 / |d|), two's complement
+      const u32 fill = 0u - (q[L] >> 31);
+      shr_to(t, q, (int)sh, fill);                // floor(. / 2^sh)
+      cadd(acc, t, sd, fill);                     // + or - by the sign of d
     }
   }
   if (!hz) {

@@ -127,21 +127,29 @@ int64_t oc_cauchy(const u32 *xm, const int64_t *xe, const int32_t *d, int K, int
   for (int i = 0; i < K; ++i) {
     const u32 *x = xm + (size_t)i * L;
     if (d[i] == 0 || bitlen(x, L) == 0) continue;
-    memcpy(a + 1, x, (size_t)L * 4); /* |x| * 2^32 */
-    a[0] = 0;
-    int sx = is_neg(a + 1, L);
-    if (sx) negate(a + 1, L);
-    u64 dd = d[i] < 0 ? (u64)(-(int64_t)d[i]) : (u64)d[i];
-    u64 rem = 0;
-    for (int k = L; k >= 0; --k) {
-      u64 cur = (rem << 32) | a[k];
-      q[k] = (u32)(cur / dd);
-      rem = cur % dd;
-    }
     int64_t sh = G + 31 - xe[i];
     if (sh >= 32 * (L + 1)) continue;
-    shr(q, L + 1, (int)sh, t, L + 1);
-    acc_add(acc, L + 3, t, L + 1, sx != (d[i] < 0));
+    /* floor(m * 2^32 / |d|) of the signed mantissa: long division with a signed top limb, remainders >= 0 */
+    int64_t dd = d[i] < 0 ? -(int64_t)d[i] : (int64_t)d[i];
+    int64_t cur = (int32_t)x[L - 1];
+    int64_t qt = cur >= 0 ? cur / dd : -((-cur + dd - 1) / dd);
+    int64_t rem = cur - qt * dd;
+    q[L] = (u32)(int32_t)qt;
+    for (int k = L - 1; k >= 0; --k) {
+      u64 c = ((u64)rem << 32) | (k > 0 ? x[k - 1] : 0u);
+      q[k] = (u32)(c / (u64)dd);
+      rem = (int64_t)(c % (u64)dd);
+    }
+    /* arithmetic shift right by sh (floor), sign-extended into L + 3 limbs, added with the sign of d */
+    u32 ext[OC_MAXL + 3];
+    const u32 fill = (q[L] >> 31) ? 0xFFFFFFFFu : 0u;
+    for (int k = 0; k < L + 3; ++k) {
+      int64_t bit = (int64_t)k * 32 + sh;
+      int w = (int)(bit >> 5), r = (int)(bit & 31);
+      u64 lo = w < L + 1 ? q[w] : fill, hi = w + 1 < L + 1 ? q[w + 1] : fill;
+      ext[k] = r ? (u32)((lo >> r) | (hi << (32 - r))) : (u32)lo;
+    }
+    acc_add(acc, L + 3, ext, L + 3, d[i] < 0);
   }
   if (hnz) {
     memcpy(a, hm, (size_t)L * 4);

@@ -143,9 +143,9 @@ def cauchy(xs, ds, L, h=(0, EZ)):
     """h + sum_i x_i * 2 / d_i for small nonzero integers d_i (d_i == 0: no term), one rounding.
     E = the largest exponent of the nonzero x_i (all columns of the group, whatever d); the grid is
     G = max(E - 31, e_h - 32) (E - 31 if h is zero, e_h - 32 if every x_i is zero); term i is
-    sign(x_i d_i) * floor(|x_i| * 2^(32 - sh_i) / |d_i|), sh_i = G + 31 - e_i >= 0, and h is truncated
-    toward zero onto 2^G (|m_h| 2^(e_h - G), e_h - G <= 32); the terms and h are summed exactly and
-    rounded once by norm."""
+    sign(d_i) * floor(m_i * 2^(32 - sh_i) / |d_i|), sh_i = G + 31 - e_i >= 0, with the signed mantissa m_i
+    and floor toward -infinity (no term when sh_i >= 32 (L + 1)); h is truncated toward zero onto 2^G
+    (|m_h| 2^(e_h - G), e_h - G <= 32); the terms and h are summed exactly and rounded once by norm."""
     nz = [e for m, e in xs if m != 0]
     mh, eh = h
     if not nz and mh == 0:
@@ -161,8 +161,10 @@ def cauchy(xs, ds, L, h=(0, EZ)):
         if d == 0 or m == 0:
             continue
         sh = G + 31 - e
-        q = (abs(m) << 32) // (abs(d) << sh) if sh <= 32 * L + 64 else 0
-        acc += q if (m < 0) == (d < 0) else -q
+        if sh >= 32 * (L + 1):
+            continue
+        q = (m << 32) // (abs(d) << sh)
+        acc += q if d > 0 else -q
     if mh:
         t = eh - G
         a = abs(mh) << t if t >= 0 else abs(mh) >> (-t)

@@ -97,16 +97,21 @@ def test_norm_widths(lib, L):
 
 @pytest.mark.parametrize("L", LS)
 def test_div_small_exact(lib, L):
+    """floor(m 2^32 / d) of a signed L-limb mantissa, every quotient digit exact (floor toward -infinity)"""
     rng = random.Random(200 + L)
     ds = [1, 2, 3, 7, 255, 256, 1000, (1 << 15) - 1, (1 << 15) - 2, (1 << 14) + 1]
-    for i in range(300):
+    for i in range(600):
         d = ds[i % len(ds)] if i < 3 * len(ds) else rng.randint(1, (1 << 15) - 1)
         nb = R.NB(L)
-        x = rng.choice([rng.getrandbits(nb), (1 << nb) - 1, (1 << (32 * L)) - 1, d * rng.getrandbits(nb - 16),
-                        d * ((1 << (32 * L - 15)) - 1)])
+        x = rng.choice([rng.getrandbits(nb), (1 << nb) - 1, (1 << (32 * L - 1)) - 1, d * rng.getrandbits(nb - 16),
+                        d * ((1 << (32 * L - 16)) - 1), 1, 0])
+        if i % 2:
+            x = -x
+        if i % 97 == 5:
+            x = -(1 << (32 * L - 1))                                   # the most negative L-limb value
         q = np.zeros(L + 1, np.uint32)
         lib.hc_div(L, ptr(limbs(x, L)), d, ptr(q))
-        assert R.from_limbs(list(q) + [0]) == (x << 32) // d, (d, i)
+        assert R.from_limbs(list(q)) == (x << 32) // d, (d, i, x)
 
 
 @pytest.mark.parametrize("L", [4, 34])
