@@ -124,6 +124,19 @@ __global__ __launch_bounds__(256) void k_align_rows(const int32_t* __restrict__ 
 
 // ---------------------------------------------------------------------------------------------- C-ABI
 
+// 28-bit magnitude limbs + sign mask of each entry (the pole step's constant operand form)
+template <int L>
+__global__ __launch_bounds__(256) void k_to28(const int32_t* __restrict__ m, i64 ps, i64 n, int32_t* __restrict__ o28,
+                                              i64 ops, uint32_t* __restrict__ osg) {
+  constexpr int NL = mp::R28<L>::NL;
+  for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
+    u32 a[L], a28[NL];
+    mp::load(a, m, ps, i);
+    osg[i] = mp::split28<L>(a, a28);
+    mp::store(a28, o28, ops, i);
+  }
+}
+
 extern "C" {
 
 const char* b3d_last_error(void) { return b3d::g_err.c_str(); }
@@ -152,6 +165,17 @@ int b3d_mpf_norm_wide(int L, int LW, const int32_t* W, int64_t wps, const int64_
   hipStream_t s = (hipStream_t)stream;
   B3D_DISPATCH_L(L, k_norm_wide<LL><<<dim3(grid_for(n, 256)), dim3(256), 0, s>>>(W, wps, LW, ein, om, oe, ops, n));
   return b3d::check_launch("b3d_mpf_norm_wide");
+}
+
+int b3d_limbs28(int L) { return b3d_supported_limbs(L) ? (32 * L - 2 + 27) / 28 : -1; }
+
+int b3d_mpf_to28(int L, const int32_t* m, int64_t ps, int64_t n, int32_t* o28, int64_t ops, uint32_t* osg,
+                 void* stream) {
+  if (n <= 0) return 0;
+  if (b3d::check_planes("b3d_mpf_to28", {ps, ops})) return 2;
+  hipStream_t s = (hipStream_t)stream;
+  B3D_DISPATCH_L(L, k_to28<LL><<<dim3(grid_for(n, 256)), dim3(256), 0, s>>>(m, ps, n, o28, ops, osg));
+  return b3d::check_launch("b3d_mpf_to28");
 }
 
 int b3d_mpf_add(int L, const int32_t* am, const int64_t* ae, int64_t aps, int64_t sa, const int32_t* bm,

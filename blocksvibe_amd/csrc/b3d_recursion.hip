@@ -10,7 +10,8 @@
 //
 // Layouts (limb planes, entries innermost = right-hand side k):
 //   slot buffer res: (L, nslots, S*S, B), H: (L, nH, S*S, B), Y: (L, T, S, S, B),
-//   Ralign / QLalign: (L, ntab, S, S) mantissas aligned per row, (ntab, S) row exponents.
+//   R / qL constants: aligned per row, as 28-bit magnitude limb planes (NL, ntab, S, S) + sign masks
+//   (ntab, S, S) (b3d_mpf_to28), (ntab, S) row exponents.
 #include "common.hpp"
 #include "mp.hpp"
 #include "../../include/b3dhip.h"
@@ -45,9 +46,9 @@ __global__ __launch_bounds__(256) void k_cauchy(const int32_t* __restrict__ rm, 
 template <int L>
 __global__ __launch_bounds__(256) void k_pole_r(const int32_t* __restrict__ Hm, const i64* __restrict__ He, i64 hps,
                                                 const int64_t* __restrict__ thrw, const int32_t* __restrict__ tR,
-                                                i64 T, int S, i64 B, const int32_t* __restrict__ Rm,
-                                                const i64* __restrict__ Re, i64 rps, int32_t* __restrict__ Ym,
-                                                i64* __restrict__ Ye, i64 yps) {
+                                                i64 T, int S, i64 B, const int32_t* __restrict__ R28,
+                                                const uint32_t* __restrict__ Rsg, const i64* __restrict__ Re,
+                                                i64 rps, int32_t* __restrict__ Ym, i64* __restrict__ Ye, i64 yps) {
   const i64 SB = (i64)S * B;
   for (i64 t = blockIdx.x * (i64)blockDim.x + threadIdx.x; t < T * S * SB; t += (i64)gridDim.x * blockDim.x) {
     const i64 k = t % B;
@@ -57,7 +58,7 @@ __global__ __launch_bounds__(256) void k_pole_r(const int32_t* __restrict__ Hm, 
     const i64 r = tR[task];
     const i64 xbase = thrw[task] * S * SB + a * SB + k;  // H[hrow][a*S + b][k], b stride B
     u32 out[L];
-    const i64 e = mp::small_dot<L>(Rm, rps, (r * S + b0) * S, Re[r * S + b0], Hm, He, hps, xbase, B, S, out);
+    const i64 e = mp::small_dot<L>(R28, Rsg, rps, (r * S + b0) * S, Re[r * S + b0], Hm, He, hps, xbase, B, S, out);
     mp::store(out, Ym, yps, t);  // (task, b0, a, k)
     Ye[t] = e;
   }
@@ -66,9 +67,9 @@ __global__ __launch_bounds__(256) void k_pole_r(const int32_t* __restrict__ Hm, 
 template <int L>
 __global__ __launch_bounds__(256) void k_pole_l(const int32_t* __restrict__ Ym, const i64* __restrict__ Ye, i64 yps,
                                                 const int32_t* __restrict__ tQL, const int64_t* __restrict__ tslot,
-                                                i64 T, int S, i64 B, const int32_t* __restrict__ Qm,
-                                                const i64* __restrict__ Qe, i64 qps, int32_t* __restrict__ rm,
-                                                i64* __restrict__ re, i64 rps) {
+                                                i64 T, int S, i64 B, const int32_t* __restrict__ Q28,
+                                                const uint32_t* __restrict__ Qsg, const i64* __restrict__ Qe, i64 qps,
+                                                int32_t* __restrict__ rm, i64* __restrict__ re, i64 rps) {
   const i64 SB = (i64)S * B;
   for (i64 t = blockIdx.x * (i64)blockDim.x + threadIdx.x; t < T * S * SB; t += (i64)gridDim.x * blockDim.x) {
     const i64 k = t % B;
@@ -78,7 +79,7 @@ __global__ __launch_bounds__(256) void k_pole_l(const int32_t* __restrict__ Ym, 
     const i64 q = tQL[task];
     const i64 xbase = (task * S + b0) * SB + k;  // Y[task][b0][a][k], a stride B
     u32 out[L];
-    const i64 e = mp::small_dot<L>(Qm, qps, (q * S + a0) * S, Qe[q * S + a0], Ym, Ye, yps, xbase, B, S, out);
+    const i64 e = mp::small_dot<L>(Q28, Qsg, qps, (q * S + a0) * S, Qe[q * S + a0], Ym, Ye, yps, xbase, B, S, out);
     const i64 o = tslot[task] * S * SB + (a0 * S + b0) * B + k;
     mp::store(out, rm, rps, o);
     re[o] = e;
@@ -100,9 +101,9 @@ int b3d_cauchy_level(int L, const int32_t* rm, const int64_t* re, int64_t rps, i
 
 int b3d_pole_level(int L, int S, int64_t B, int64_t T, const int32_t* Hm, const int64_t* He, int64_t hps,
                    const int64_t* thrw, const int32_t* tR, const int32_t* tQL, const int64_t* tslot,
-                   const int32_t* Rm, const int64_t* Re, int64_t rtps, const int32_t* Qm, const int64_t* Qe,
-                   int64_t qps, int32_t* Ym, int64_t* Ye, int64_t yps, int32_t* rm, int64_t* re, int64_t rps,
-                   void* stream) {
+                   const int32_t* R28, const uint32_t* Rsg, const int64_t* Re, int64_t rtps, const int32_t* Q28,
+                   const uint32_t* Qsg, const int64_t* Qe, int64_t qps, int32_t* Ym, int64_t* Ye, int64_t yps,
+                   int32_t* rm, int64_t* re, int64_t rps, void* stream) {
   if (T <= 0 || B <= 0) return 0;
   if (S < 1 || S > 16) {
     b3d::set_error("b3d_pole_level: structure count S must be 1..16");
@@ -111,9 +112,9 @@ int b3d_pole_level(int L, int S, int64_t B, int64_t T, const int32_t* Hm, const 
   if (b3d::check_planes("b3d_pole_level", {hps, rtps, qps, yps, rps})) return 2;
   hipStream_t s = (hipStream_t)stream;
   const int64_t n = T * S * S * B;
-  B3D_DISPATCH_L(L, k_pole_r<LL><<<dim3(grid_for(n, 256)), dim3(256), 0, s>>>(Hm, He, hps, thrw, tR, T, S, B, Rm, Re, rtps, Ym, Ye, yps));
+  B3D_DISPATCH_L(L, k_pole_r<LL><<<dim3(grid_for(n, 256)), dim3(256), 0, s>>>(Hm, He, hps, thrw, tR, T, S, B, R28, Rsg, Re, rtps, Ym, Ye, yps));
   if (b3d::check_launch("b3d_pole_level (R)")) return 1;
-  B3D_DISPATCH_L(L, k_pole_l<LL><<<dim3(grid_for(n, 256)), dim3(256), 0, s>>>(Ym, Ye, yps, tQL, tslot, T, S, B, Qm, Qe, qps, rm, re, rps));
+  B3D_DISPATCH_L(L, k_pole_l<LL><<<dim3(grid_for(n, 256)), dim3(256), 0, s>>>(Ym, Ye, yps, tQL, tslot, T, S, B, Q28, Qsg, Qe, qps, rm, re, rps));
   return b3d::check_launch("b3d_pole_level (L)");
 }
 

@@ -276,6 +276,11 @@ __device__ __forceinline__ void store(const u32 (&a)[N], int32_t* __restrict__ p
 __device__ __forceinline__ u32 load_limb(const int32_t* __restrict__ p, i64 ps, int k, i64 idx) {
   return (u32)p[k * ps + idx];
 }
+
+template <int N>
+__device__ __forceinline__ void load_desc(u32 (&a)[N], const int32_t* __restrict__ p, i64 ps, i64 idx) {
+  for (int k = N - 1; k >= 0; --k) a[k] = (u32)p[k * ps + idx];
+}
 #else
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const char* base) {
   // raw buffer (stride 0), every 32-bit offset in range, 32-bit data format (dword 3 = 0x00020000)
@@ -310,6 +315,21 @@ __device__ __forceinline__ void store(const u32 (&a)[N], int32_t* __restrict__ p
 
 __device__ __forceinline__ u32 load_limb(const int32_t* __restrict__ p, i64 ps, int k, i64 idx) {
   return __builtin_amdgcn_raw_buffer_load_b32(plane_rsrc((const char*)p + k * ps * 4), (u32)idx * 4u, 0, 0);
+}
+
+// the same load issued from the top limb down, for consumers that start with the top limb (the long division):
+// the first limbs they need return first
+template <int N>
+__device__ __forceinline__ void load_desc(u32 (&a)[N], const int32_t* __restrict__ p, i64 ps, i64 idx) {
+  const u32 off = (u32)idx * 4u;
+  const i64 step = ps * 4;
+  const char* base = (const char*)p + (N - 1) * step;
+#pragma unroll
+  for (int k = N - 1; k >= 0; --k) {
+    a[k] = __builtin_amdgcn_raw_buffer_load_b32(plane_rsrc(base), off, 0, 0);
+    base -= step;
+    asm volatile("" : "+s"(base));
+  }
 }
 #endif
 
@@ -450,12 +470,14 @@ __device__ __forceinline__ void pack28(const int32_t (&acc)[NA], u32 (&W)[NW]) {
   for (int w = W0 + 2; w < NW; ++w) W[w] = ext;
 }
 
-// out = norm(sum_j A[j] * X[j]) over K = S terms: A pre-aligned (mantissas a_j, exponent ea), X entries
-// aligned here (over j).  Shared by both pole-step products.  The exact sum is accumulated in 28-bit limbs.
+// out = norm(sum_j A[j] * X[j]) over K = S terms: A pre-aligned and stored as 28-bit magnitude limbs (NL limb
+// planes a28, plane stride aps) with sign masks asg (exponent ea), X entries aligned here (over j).  Shared
+// by both pole-step products.  The exact sum is accumulated in 28-bit limbs.
 template <int L>
-__device__ __forceinline__ i64 small_dot(const int32_t* __restrict__ am, i64 aps, i64 abase, i64 ea,
-                                         const int32_t* __restrict__ xm, const i64* __restrict__ xe, i64 xps,
-                                         i64 xbase, i64 xstride, int S, u32 (&out)[L]) {
+__device__ __forceinline__ i64 small_dot(const int32_t* __restrict__ a28, const uint32_t* __restrict__ asg, i64 aps,
+                                         i64 abase, i64 ea, const int32_t* __restrict__ xm,
+                                         const i64* __restrict__ xe, i64 xps, i64 xbase, i64 xstride, int S,
+                                         u32 (&out)[L]) {
   constexpr int NL = R28<L>::NL;
   i64 E = B3D_EZ;
   for (int j = 0; j < S; ++j) {
@@ -481,17 +503,29 @@ __device__ __forceinline__ i64 small_dot(const int32_t* __restrict__ am, i64 aps
     const i64 sh = E - xe[idx];
     if (sh >= 32 * L) continue;
     mp::shr(x, (int)sh);
-    u32 a[L];
-    mp::load(a, am, aps, abase + j);
-    const u32 sa = mp::to_mag(a);
-    int32_t a28[NL], y28[NL];
-    to28(a, 0u, a28);
-    to28(x, sx ^ sa, y28);
-    mac28(acc, a28, y28);
+    u32 au[NL];
+    mp::load(au, a28, aps, abase + j);
+    int32_t a[NL], y28[NL];
+#pragma unroll
+    for (int k = 0; k < NL; ++k) a[k] = (int32_t)au[k];
+    to28(x, sx ^ asg[abase + j], y28);
+    mac28(acc, a, y28);
   }
   u32 W[2 * L + 2];
   pack28(acc, W);
   return mp::norm<L, 2 * L + 2>(W, ea + E, out);
+}
+
+// a two's complement value of L limbs as 28-bit magnitude limbs and a sign mask (the pole step's constant form)
+template <int L>
+__device__ __forceinline__ u32 split28(u32 (&a)[L], u32 (&a28)[R28<L>::NL]) {
+  constexpr int NL = R28<L>::NL;
+  const u32 sa = to_mag(a);
+  int32_t t[NL];
+  to28(a, 0u, t);
+#pragma unroll
+  for (int k = 0; k < NL; ++k) a28[k] = (u32)t[k];
+  return sa;
 }
 
 // out = norm(a * b) for two's complement a, b of L limbs (both destroyed), exponent e of the product
@@ -508,6 +542,21 @@ __device__ __forceinline__ i64 mul_norm(u32 (&a)[L], u32 (&b)[L], i64 e, u32 (&o
   u32 W[2 * L + 2];
   pack28(acc, W);
   return norm<L, 2 * L + 2>(W, e, out);
+}
+
+// One Cauchy term on the grid 2^G: acc += sign(d) floor(m 2^32 / |d| / 2^(G + 31 - e)) for the signed
+// mantissa m (limbs x) with exponent e; nothing for d == 0, m == 0 or a shift of 32 (L + 1) or more.
+template <int L>
+__device__ __forceinline__ void cauchy_term(u32 (&acc)[L + 3], const u32 (&x)[L], i64 e, int d, i64 G) {
+  if (d == 0 || x[L - 1] == 0u) return;
+  const i64 sh = G + 31 - e;
+  if (sh >= 32 * (L + 1)) return;
+  const u32 sd = 0u - (u32)(d < 0);
+  u32 q[L + 1], t[L + 1];
+  div_small<L>(x, (u32)(d < 0 ? -d : d), q);   // floor(m 2^32 / |d|), two's complement
+  const u32 fill = 0u - (q[L] >> 31);
+  shr_to(t, q, (int)sh, fill);                 // floor(. / 2^sh)
+  cadd(acc, t, sd, fill);                      // + or - by the sign of d
 }
 
 // One Cauchy-step entry: h + sum_i x_i * 2 / (D - D_i) for column n of a group whose prime poles are the
@@ -544,21 +593,12 @@ __device__ __forceinline__ i64 cauchy_entry(const int32_t* __restrict__ rm, cons
   zero(acc);
   if (E != B3D_EZ) {
     for (int i = 0; i < nc; ++i) {
-      int d = D - coldg[i];
+      const int d = D - coldg[i];
       if (d == 0) continue;
       const i64 idx = colg[i] * N + n;
       u32 x[L];
-      load(x, rm, rps, idx);
-      if (x[L - 1] == 0u) continue;
-      const i64 sh = G + 31 - re[idx];
-      if (sh >= 32 * (L + 1)) continue;
-      const u32 sd = 0u - (u32)(d < 0);
-      d = d < 0 ? -d : d;
-      u32 q[L + 1], t[L + 1];
-      div_small<L>(x, (u32)d, q);                 // floor(m 2^32 / |d|), two's complement
-      const u32 fill = 0u - (q[L] >> 31);
-      shr_to(t, q, (int)sh, fill);                // floor(. / 2^sh)
-      cadd(acc, t, sd, fill);                     // + or - by the sign of d
+      load_desc(x, rm, rps, idx);
+      cauchy_term<L>(acc, x, re[idx], d, G);
     }
   }
   if (!hz) {

@@ -33,6 +33,16 @@ def number_pair(x, L):
     return mpf.from_mpmath(x, L)
 
 
+def _split28(m, L):
+    """row-aligned mantissas (L limb planes) -> (28-bit magnitude limb planes, sign masks), on the GPU"""
+    n = m[0].numel()
+    nl = _lib.lib().b3d_limbs28(L)
+    o28 = torch.empty((nl,) + tuple(m.shape[1:]), dtype=torch.int32, device=m.device)
+    sg = torch.empty(tuple(m.shape[1:]), dtype=torch.int32, device=m.device)
+    _lib.call("b3d_mpf_to28", L, m.data_ptr(), n, n, o28.data_ptr(), n, sg.data_ptr(), mpf.stream_ptr())
+    return o28, sg
+
+
 class Recursion:
     """Device constants and level tables of one Plan at L limbs.
     M: dict {Mkey: matrix (rows of numbers)} for every key of plan.Mkeys (absent keys: zero matrices), an
@@ -94,6 +104,9 @@ class Recursion:
                      q.e[:, None, None].expand(len(lq), Sm, Sm).contiguous())
         self.QL = mpf.mul(Lm, qx)
         self.QLalign_m, self.QLalign_e = mpf.align_rows(self.QL)
+        # the pole step reads both constant operands as 28-bit magnitude limbs + sign masks
+        self.R28, self.Rsg = _split28(self.Ralign_m, L)
+        self.QL28, self.QLsg = _split28(self.QLalign_m, L)
 
     def _upload_levels(self):
         plan, d = self.plan, mpf.dev()
@@ -161,8 +174,8 @@ class Recursion:
                 t1 = time.time()
                 prof["cauchy"] = prof.get("cauchy", 0.0) + t1 - t0
             _lib.call("b3d_pole_level", L, Sm, B, lv["T"], P(H.m), P(H.e), H.numel(), P(lv["t_hrow"]),
-                      P(lv["t_R"]), P(lv["t_qL"]), P(lv["t_slot"]), P(self.Ralign_m), P(self.Ralign_e),
-                      self.Ralign_m[0].numel(), P(self.QLalign_m), P(self.QLalign_e), self.QLalign_m[0].numel(),
+                      P(lv["t_R"]), P(lv["t_qL"]), P(lv["t_slot"]), P(self.R28), P(self.Rsg), P(self.Ralign_e),
+                      self.R28[0].numel(), P(self.QL28), P(self.QLsg), P(self.QLalign_e), self.QL28[0].numel(),
                       P(Y.m), P(Y.e), Y.numel(), P(res.m), P(res.e), res.numel(), st)
             if prof is not None:
                 torch.cuda.synchronize()

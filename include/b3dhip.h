@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-#define B3D_ABI_VERSION 4
+#define B3D_ABI_VERSION 5
 
 /* Last error message of the calling thread (empty if none). */
 const char* b3d_last_error(void);
@@ -42,6 +42,16 @@ int b3d_mpf_add(int L, const int32_t* am, const int64_t* ae, int64_t aps, int64_
 int b3d_mpf_mul(int L, const int32_t* am, const int64_t* ae, int64_t aps, int64_t sa, const int32_t* bm,
                 const int64_t* be, int64_t bps, int64_t sb, int32_t* cm, int64_t* ce, int64_t cps, int64_t n,
                 void* stream);
+/* Number of 28-bit limbs of a magnitude below 2^(32L - 2) (the pole step's constant form), -1 if L has no
+ * kernels. */
+int b3d_limbs28(int L);
+
+/* Constant operands of the pole step: each of the n entries m (two's complement, plane stride ps) as
+ * b3d_limbs28(L) 28-bit magnitude limbs (o28, plane stride ops) and a sign mask osg (0 or ~0).  No
+ * counterpart in the reference (its smallgemm re-splits the operands on every call). */
+int b3d_mpf_to28(int L, const int32_t* m, int64_t ps, int64_t n, int32_t* o28, int64_t ops, uint32_t* osg,
+                 void* stream);
+
 /* Align each row of a (rows, K) array (entry (r,k) at r*rs + k*ks) to its largest exponent: aligned
  * mantissas to om (contiguous (rows, K), plane stride ops), row exponents to oe[rows]. */
 int b3d_mpf_align_rows(int L, const int32_t* m, const int64_t* e, int64_t ps, int64_t rows, int64_t K,
@@ -60,13 +70,15 @@ int b3d_cauchy_level(int L, const int32_t* rm, const int64_t* re, int64_t rps, i
                      int64_t* oe, int64_t ops, void* stream);
 
 /* Pole step for T tasks: Y = Ralign[tR[t]] . H[thrw[t]]^T, then res[tslot[t]] = QLalign[tQL[t]] . Y^T,
- * S x S structure matrices, B right-hand sides.  Ralign/QLalign: (L, ntab, S, S) row-aligned mantissas
- * (plane strides rtps/qps) with row exponents (ntab, S).  Y: scratch (L, T, S, S, B), plane stride yps. */
+ * S x S structure matrices, B right-hand sides.  Ralign/QLalign are given in the form b3d_mpf_to28 makes of
+ * the row-aligned mantissas: 28-bit magnitude limb planes R28/Q28 (b3d_limbs28(L) planes of (ntab, S, S),
+ * plane strides rtps/qps) and sign masks Rsg/Qsg (ntab, S, S), with row exponents Re/Qe (ntab, S).
+ * Y: scratch (L, T, S, S, B), plane stride yps. */
 int b3d_pole_level(int L, int S, int64_t B, int64_t T, const int32_t* Hm, const int64_t* He, int64_t hps,
                    const int64_t* thrw, const int32_t* tR, const int32_t* tQL, const int64_t* tslot,
-                   const int32_t* Rm, const int64_t* Re, int64_t rtps, const int32_t* Qm, const int64_t* Qe,
-                   int64_t qps, int32_t* Ym, int64_t* Ye, int64_t yps, int32_t* rm, int64_t* re, int64_t rps,
-                   void* stream);
+                   const int32_t* R28, const uint32_t* Rsg, const int64_t* Re, int64_t rtps, const int32_t* Q28,
+                   const uint32_t* Qsg, const int64_t* Qe, int64_t qps, int32_t* Ym, int64_t* Ye, int64_t yps,
+                   int32_t* rm, int64_t* re, int64_t rps, void* stream);
 
 /* ---- derivative moments during the recursion (replace the momacc/momflush kernels and the staging of
  *      reference blocksvibe/derivatives.py:46-189, 474-510) ---- */

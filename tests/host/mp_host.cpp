@@ -89,8 +89,18 @@ static int64_t cauchy_t(const int32_t* rm, const int64_t* re, int64_t rps, int64
 template <int L>
 static int64_t dot_t(const int32_t* am, int64_t aps, int64_t abase, int64_t ea, const int32_t* xm, const int64_t* xe,
                      int64_t xps, int64_t xbase, int64_t xstride, int S, uint32_t* r) {
+  // the constant operand in the pole step's form: 28-bit magnitude limb planes and sign masks (as b3d_mpf_to28)
+  constexpr int NL = mp::R28<L>::NL;
+  static int32_t a28[NL * 64];
+  static uint32_t sg[64];
+  for (int j = 0; j < aps && j < 64; ++j) {
+    u32 a[L], t[NL];
+    for (int k = 0; k < L; ++k) a[k] = (u32)am[k * aps + j];
+    sg[j] = mp::split28<L>(a, t);
+    for (int k = 0; k < NL; ++k) a28[k * aps + j] = (int32_t)t[k];
+  }
   u32 R[L];
-  const int64_t e = mp::small_dot<L>(am, aps, abase, ea, xm, xe, xps, xbase, xstride, S, R);
+  const int64_t e = mp::small_dot<L>(a28, sg, aps, abase, ea, xm, xe, xps, xbase, xstride, S, R);
   copy_out(r, R);
   return e;
 }
