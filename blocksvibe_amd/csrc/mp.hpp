@@ -150,6 +150,28 @@ __device__ __forceinline__ void shr(u32 (&a)[N], int s) {
   for (int k = 0; k < N; ++k) a[k] = t[k];
 }
 
+// right shift in place by s >= 0 bits, the vacated bits filled from `fill` (the sign mask of a two's
+// complement value: floor division by 2^s).  Barrel stages as in shr_to; the funnel pass runs upward, so
+// each limb reads its upper neighbour before that one is rewritten.
+template <int N>
+__device__ __forceinline__ void sar(u32 (&a)[N], int s, u32 fill) {
+  int q = s >> 5;
+  if (q > N) q = N;
+  constexpr int TOP = (N >= 64) ? 6 : (N >= 32) ? 5 : (N >= 16) ? 4 : (N >= 8) ? 3 : (N >= 4) ? 2 : (N >= 2) ? 1 : 0;
+#pragma unroll
+  for (int b = TOP; b >= 0; --b) {
+    const int st = 1 << b;
+    const bool on = (q >> b) & 1;
+    if (B3D_ANY(on)) {
+#pragma unroll
+      for (int k = 0; k < N; ++k) a[k] = on ? (k + st < N ? a[k + st] : fill) : a[k];
+    }
+  }
+  const u32 r = (u32)(s & 31);
+#pragma unroll
+  for (int k = 0; k < N; ++k) a[k] = b3d_alignbit(k + 1 < N ? a[k + 1] : fill, a[k], r);
+}
+
 // left shift in place by s >= 0 bits (bits shifted out are lost)
 template <int N>
 __device__ __forceinline__ void shl(u32 (&a)[N], int s) {
@@ -277,9 +299,10 @@ __device__ __forceinline__ u32 load_limb(const int32_t* __restrict__ p, i64 ps, 
   return (u32)p[k * ps + idx];
 }
 
-template <int N>
+template <int NL = -1, int N>
 __device__ __forceinline__ void load_desc(u32 (&a)[N], const int32_t* __restrict__ p, i64 ps, i64 idx) {
-  for (int k = N - 1; k >= 0; --k) a[k] = (u32)p[k * ps + idx];
+  constexpr int C = NL < 0 ? N : NL;
+  for (int k = C - 1; k >= 0; --k) a[N - C + k] = (u32)p[k * ps + idx];
 }
 #else
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const char* base) {
@@ -318,15 +341,16 @@ __device__ __forceinline__ u32 load_limb(const int32_t* __restrict__ p, i64 ps, 
 }
 
 // the same load issued from the top limb down, for consumers that start with the top limb (the long division):
-// the first limbs they need return first
-template <int N>
+// the first limbs they need return first.  With NL given, the entry's NL limbs go to the top NL of a[N].
+template <int NL = -1, int N>
 __device__ __forceinline__ void load_desc(u32 (&a)[N], const int32_t* __restrict__ p, i64 ps, i64 idx) {
+  constexpr int C = NL < 0 ? N : NL;
   const u32 off = (u32)idx * 4u;
   const i64 step = ps * 4;
-  const char* base = (const char*)p + (N - 1) * step;
+  const char* base = (const char*)p + (C - 1) * step;
 #pragma unroll
-  for (int k = N - 1; k >= 0; --k) {
-    a[k] = __builtin_amdgcn_raw_buffer_load_b32(plane_rsrc(base), off, 0, 0);
+  for (int k = C - 1; k >= 0; --k) {
+    a[N - C + k] = __builtin_amdgcn_raw_buffer_load_b32(plane_rsrc(base), off, 0, 0);
     base -= step;
     asm volatile("" : "+s"(base));
   }
@@ -379,29 +403,28 @@ __device__ __forceinline__ i64 add(const u32 (&a)[L], i64 ea, const u32 (&b)[L],
   return norm<L, L + 3>(acc, tt, r);
 }
 
-// q = floor(m * 2^32 / d) for a signed mantissa m (two's complement, L limbs) and 0 < d < 2^15: L + 1
-// quotient limbs in two's complement (the top digit signed, the others in [0, 2^32)), by long division one
-// 32-bit limb at a time in FP64 (full rate on CDNA4) with remainders in [0, d).  The partial dividend
-// cur = rem 2^32 + limb (|cur| < 2^47; the top one is the signed top limb) is exact; fma(cur, 1/d, 2^-19)
-// is within 2^-20 of cur/d + 2^-19 (rounding of 1/d and of the fma, each < 2^-21 on a value below 2^32 in
-// magnitude), and the fractional part of cur/d is at most 1 - 1/d < 1 - 2^-15, so its floor is the exact
-// digit; the remainder cur - q d is exact.
-template <int L>
-__device__ __forceinline__ void div_small(const u32 (&x)[L], u32 d, u32 (&q)[L + 1]) {
+// x = floor(x / d) in place for a signed N-limb value x (two's complement) and 0 < d < 2^15: the quotient in
+// two's complement (the top digit signed, the others in [0, 2^32)), by long division one 32-bit limb at a
+// time in FP64 (full rate on CDNA4) with remainders in [0, d).  The partial dividend cur = rem 2^32 + limb
+// (|cur| < 2^47; the top one is the signed top limb) is exact; fma(cur, 1/d, 2^-19) is within 2^-20 of
+// cur/d + 2^-19 (rounding of 1/d and of the fma, each < 2^-21 on a value below 2^32 in magnitude), and the
+// fractional part of cur/d is at most 1 - 1/d < 1 - 2^-15, so its floor is the exact digit; the remainder
+// cur - q d is exact.
+template <int N>
+__device__ __forceinline__ void div_floor(u32 (&x)[N], u32 d) {
   const double dd = (double)d;
   const double rd = 1.0 / dd;
   const double bias = 1.0 / 524288.0;   // 2^-19
-  double cur = (double)(int32_t)x[L - 1];
+  double cur = (double)(int32_t)x[N - 1];
   double qd = floor(fma(cur, rd, bias));
   double rem = fma(-qd, dd, cur);
-  q[L] = (u32)(int32_t)qd;
+  x[N - 1] = (u32)(int32_t)qd;
 #pragma unroll
-  for (int k = L - 1; k >= 0; --k) {
-    const u32 limb = k > 0 ? x[k - 1] : 0u;
-    cur = fma(rem, 4294967296.0, (double)limb);
+  for (int k = N - 2; k >= 0; --k) {
+    cur = fma(rem, 4294967296.0, (double)x[k]);
     qd = floor(fma(cur, rd, bias));
     rem = fma(-qd, dd, cur);
-    q[k] = (u32)qd;
+    x[k] = (u32)qd;
   }
 }
 
@@ -544,19 +567,20 @@ __device__ __forceinline__ i64 mul_norm(u32 (&a)[L], u32 (&b)[L], i64 e, u32 (&o
   return norm<L, 2 * L + 2>(W, e, out);
 }
 
-// One Cauchy term on the grid 2^G: acc += sign(d) floor(m 2^32 / |d| / 2^(G + 31 - e)) for the signed
-// mantissa m (limbs x) with exponent e; nothing for d == 0, m == 0 or a shift of 32 (L + 1) or more.
+// One Cauchy term on the grid 2^G: acc += sign(d) floor(m 2^32 / |d| / 2^sh), sh = G + 31 - e >= 0, for the
+// signed mantissa m (limbs x) with exponent e; nothing for d == 0, m == 0 or sh >= 32 (L + 1).  The value is
+// put on the grid first, X = floor(m 2^(32 - sh)) (L + 1 limbs, arithmetic shift), then divided:
+// floor(floor(a / 2^sh) / |d|) = floor(a / (2^sh |d|)) for integer a, so the result is the same single floor,
+// and one (L + 1)-limb buffer carries the term from the load to the sum.
 template <int L>
-__device__ __forceinline__ void cauchy_term(u32 (&acc)[L + 3], const u32 (&x)[L], i64 e, int d, i64 G) {
-  if (d == 0 || x[L - 1] == 0u) return;
+__device__ __forceinline__ void cauchy_term(u32 (&acc)[L + 3], u32 (&X)[L + 1], i64 e, int d, i64 G) {
+  if (d == 0 || X[L] == 0u) return;
   const i64 sh = G + 31 - e;
   if (sh >= 32 * (L + 1)) return;
-  const u32 sd = 0u - (u32)(d < 0);
-  u32 q[L + 1], t[L + 1];
-  div_small<L>(x, (u32)(d < 0 ? -d : d), q);   // floor(m 2^32 / |d|), two's complement
-  const u32 fill = 0u - (q[L] >> 31);
-  shr_to(t, q, (int)sh, fill);                 // floor(. / 2^sh)
-  cadd(acc, t, sd, fill);                      // + or - by the sign of d
+  const u32 fill = 0u - (X[L] >> 31);
+  sar(X, (int)sh, fill);                 // floor(m 2^32 / 2^sh)
+  div_floor(X, (u32)(d < 0 ? -d : d));   // floor(. / |d|), same sign as m
+  cadd(acc, X, 0u - (u32)(d < 0), fill);  // + or - by the sign of d
 }
 
 // One Cauchy-step entry: h + sum_i x_i * 2 / (D - D_i) for column n of a group whose prime poles are the
@@ -580,9 +604,9 @@ __device__ __forceinline__ i64 cauchy_entry(const int32_t* __restrict__ rm, cons
       E = ei > E ? ei : E;
     }
   }
-  u32 hv[L];
-  load(hv, hm, hps, hidx);
-  const bool hz = hv[L - 1] == 0u;
+  // only the top limb of h here: its other limbs are loaded after the terms, so they hold no registers
+  // through the loop
+  const bool hz = load_limb(hm, hps, L - 1, hidx) == 0u;
   const i64 eh = he[hidx];
   if (E == B3D_EZ && hz) {
     zero(r);
@@ -596,19 +620,20 @@ __device__ __forceinline__ i64 cauchy_entry(const int32_t* __restrict__ rm, cons
       const int d = D - coldg[i];
       if (d == 0) continue;
       const i64 idx = colg[i] * N + n;
-      u32 x[L];
-      load_desc(x, rm, rps, idx);
-      cauchy_term<L>(acc, x, re[idx], d, G);
+      u32 X[L + 1];   // m 2^32
+      X[0] = 0u;
+      load_desc<L>(X, rm, rps, idx);
+      cauchy_term<L>(acc, X, re[idx], d, G);
     }
   }
   if (!hz) {
-    const u32 sh_ = to_mag(hv);
+    // |m_h| 2^(e_h - G) truncated = floor(|m_h| 2^32 / 2^(32 - e_h + G)), e_h - G <= 32
     u32 y[L + 1];
-#pragma unroll
-    for (int k = 0; k < L + 1; ++k) y[k] = k < L ? hv[k] : 0u;
-    const i64 t = eh - G;   // <= 32
-    if (t >= 0) shl(y, (int)t);
-    else shr(y, (-t) > 32 * (L + 1) ? 32 * (L + 1) : (int)(-t));
+    y[0] = 0u;
+    load_desc<L>(y, hm, hps, hidx);
+    const u32 sh_ = to_mag(y);
+    const i64 s = 32 - (eh - G);
+    sar(y, s > 32 * (L + 1) ? 32 * (L + 1) : (int)s, 0u);
     cadd(acc, y, sh_);
   }
   return norm<L, L + 3>(acc, G, r);

@@ -60,17 +60,19 @@ template <int L> static int64_t norm_t(const uint32_t* w, int LW, int64_t e, uin
 }
 
 template <int L> static int64_t divs_t(const uint32_t* x, uint32_t d, uint32_t* q) {
-  u32 X[L], Q[L + 1];
-  copy_in(X, x);
-  mp::div_small<L>(X, d, Q);
-  for (int k = 0; k < L + 1; ++k) q[k] = Q[k];
+  u32 X[L + 1];   // m 2^32
+  X[0] = 0u;
+  for (int k = 0; k < L; ++k) X[k + 1] = x[k];
+  mp::div_floor(X, d);
+  for (int k = 0; k < L + 1; ++k) q[k] = X[k];
   return 0;
 }
 
 template <int L> static int64_t shr_t(const uint32_t* x, int s, int left, uint32_t* out) {
   u32 X[L + 3];
   for (int k = 0; k < L + 3; ++k) X[k] = x[k];
-  if (left) mp::shl(X, s);
+  if (left == 1) mp::shl(X, s);
+  else if (left == 2) mp::sar(X, s, 0u - (X[L + 2] >> 31));
   else mp::shr(X, s);
   for (int k = 0; k < L + 3; ++k) out[k] = X[k];
   return 0;

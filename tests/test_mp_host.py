@@ -1,7 +1,7 @@
 """The gfx950 kernels' per-thread multiprecision helpers (blocksvibe_amd/csrc/mp.hpp), compiled for the
 host CPU (tests/host/mp_host.cpp), against the oracle bit for bit: add, mul (28-bit limb products), norm at
 every accumulator width the kernels use, the FP64 one-limb division (every digit exact, divisors up to
-2^15 - 1), barrel shifts, Cauchy entries with h_inf, pole-step dot products and moment windows.  CPU only;
+2^15 - 1), barrel shifts (logical and arithmetic), Cauchy entries with h_inf, pole-step dot products and moment windows.  CPU only;
 the same source is what the GPU runs."""
 import ctypes
 import os
@@ -126,6 +126,9 @@ def test_shifts(lib, L):
         assert R.from_limbs(list(out) + [0]) == x >> s
         lib.hc_shift(L, ptr(limbs(x, W)), s, 1, ptr(out))
         assert R.from_limbs(list(out) + [0]) == (x << s) & ((1 << (32 * W)) - 1)
+        lib.hc_shift(L, ptr(limbs(x, W)), s, 2, ptr(out))                # arithmetic: floor(x / 2^s)
+        xs = x - (1 << (32 * W)) if x >> (32 * W - 1) else x
+        assert R.from_limbs(list(out)) == xs >> s, (i, s)
 
 
 @pytest.mark.parametrize("L", [4, 10, 34])
