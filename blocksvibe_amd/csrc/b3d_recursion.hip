@@ -4,7 +4,8 @@
 //   k_cauchy   H[h, n] = h_inf[g(h), n] + sum_i x_i * 2 / (D'_h - D_i), x_i = res[col[g, i], n]:
 //              gathers the prime-pole residues straight from the slot buffer (no staging copy), aligns
 //              them per column, divides by the one-limb integer D' - D_i, sums exactly, rounds once and
-//              adds the h_inf entry;
+//              adds the h_inf entry (k_cauchy_rows when the rows, padded to whole waves, keep at least
+//              3/4 of the lanes busy: the group's data then sits in scalar registers; k_cauchy otherwise);
 //   k_pole_r   Y[t, b0, a, k] = sum_b Ralign[R_t][b0][b] * H[hrow_t][a, b][k]
 //   k_pole_l   res[slot_t][a0, b0][k] = sum_a QLalign[qL_t][a0][a] * Y[t, b0, a, k]
 //
@@ -23,6 +24,31 @@ static inline unsigned grid_for(int64_t n, int th) {
   return (unsigned)b;
 }
 
+// Row-uniform form: each row's N columns are laid over W >= N lanes, W a multiple of 64 (lanes n >= N idle),
+// so every wave lies in one row h and takes the group's column list, divisors and plane bases from scalar
+// registers.  Up to L = 34 that leaves at most 96 VGPRs, five waves per SIMD (the compiler is asked for that
+// occupancy; the term loop waits on its limb loads, so more waves hide more of that latency).
+template <int L>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L <= 34 ? 5 : 1))) void k_cauchy_rows(
+    const int32_t* __restrict__ rm, const i64* __restrict__ re, i64 rps, i64 N, const int32_t* __restrict__ hm,
+    const i64* __restrict__ he, i64 hps, const int64_t* __restrict__ ghrow, const int32_t* __restrict__ rg,
+    const int32_t* __restrict__ rd, i64 nH, const int64_t* __restrict__ col, const int32_t* __restrict__ cold,
+    const int32_t* __restrict__ ncol, int Kc, int32_t* __restrict__ om, i64* __restrict__ oe, i64 ops, i64 W) {
+  for (i64 u = blockIdx.x * (i64)blockDim.x + threadIdx.x; u < nH * W; u += (i64)gridDim.x * blockDim.x) {
+    const i64 n = u % W;
+    const int h = __builtin_amdgcn_readfirstlane((int)(u / W));
+    if (n >= N) continue;
+    const int g = rg[h];
+    u32 r[L];
+    const i64 er = mp::cauchy_entry<L, true>(rm, re, rps, N, n, col + (i64)g * Kc, cold + (i64)g * Kc, ncol[g],
+                                             rd[h], hm, he, hps, ghrow[g], r);
+    const i64 t = (i64)h * N + n;
+    mp::store(r, om, ops, t);
+    oe[t] = er;
+  }
+}
+
+// General form (rows that would leave many lanes idle if padded to whole waves): every index per lane.
 template <int L>
 __global__ __launch_bounds__(256) void k_cauchy(const int32_t* __restrict__ rm, const i64* __restrict__ re, i64 rps,
                                                 i64 N, const int32_t* __restrict__ hm, const i64* __restrict__ he,
@@ -34,10 +60,9 @@ __global__ __launch_bounds__(256) void k_cauchy(const int32_t* __restrict__ rm, 
   for (i64 t = blockIdx.x * (i64)blockDim.x + threadIdx.x; t < nH * N; t += (i64)gridDim.x * blockDim.x) {
     const i64 n = t % N, h = t / N;
     const int g = rg[h];
-    const int D = rd[h];
     u32 r[L];
-    const i64 er = mp::cauchy_entry<L>(rm, re, rps, N, n, col + (i64)g * Kc, cold + (i64)g * Kc, ncol[g], D, hm, he,
-                                       hps, ghrow[g] * N + n, r);
+    const i64 er = mp::cauchy_entry<L, false>(rm, re, rps, N, n, col + (i64)g * Kc, cold + (i64)g * Kc, ncol[g],
+                                              rd[h], hm, he, hps, ghrow[g], r);
     mp::store(r, om, ops, t);
     oe[t] = er;
   }
@@ -95,7 +120,13 @@ int b3d_cauchy_level(int L, const int32_t* rm, const int64_t* re, int64_t rps, i
   if (nH <= 0 || N <= 0) return 0;
   if (b3d::check_planes("b3d_cauchy_level", {rps, hps, ops})) return 2;
   hipStream_t s = (hipStream_t)stream;
-  B3D_DISPATCH_L(L, k_cauchy<LL><<<dim3(grid_for(nH * N, 256)), dim3(256), 0, s>>>(rm, re, rps, N, hm, he, hps, ghrow, rg, rd, nH, col, cold, ncol, Kc, om, oe, ops));
+  // row-uniform kernel unless padding the rows to whole waves would idle more than a quarter of the lanes
+  const int64_t W = (N + 63) / 64 * 64;
+  if (4 * (W - N) <= W) {
+    B3D_DISPATCH_L(L, k_cauchy_rows<LL><<<dim3(grid_for(nH * W, 256)), dim3(256), 0, s>>>(rm, re, rps, N, hm, he, hps, ghrow, rg, rd, nH, col, cold, ncol, Kc, om, oe, ops, W));
+  } else {
+    B3D_DISPATCH_L(L, k_cauchy<LL><<<dim3(grid_for(nH * N, 256)), dim3(256), 0, s>>>(rm, re, rps, N, hm, he, hps, ghrow, rg, rd, nH, col, cold, ncol, Kc, om, oe, ops));
+  }
   return b3d::check_launch("b3d_cauchy_level");
 }
 

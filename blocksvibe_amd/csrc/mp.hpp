@@ -573,7 +573,7 @@ __device__ __forceinline__ i64 mul_norm(u32 (&a)[L], u32 (&b)[L], i64 e, u32 (&o
 // floor(floor(a / 2^sh) / |d|) = floor(a / (2^sh |d|)) for integer a, so the result is the same single floor,
 // and one (L + 1)-limb buffer carries the term from the load to the sum.
 template <int L>
-__device__ __forceinline__ void cauchy_term(u32 (&acc)[L + 3], u32 (&X)[L + 1], i64 e, int d, i64 G) {
+__device__ __forceinline__ void cauchy_term(u32 (&acc)[L + 2], u32 (&X)[L + 1], i64 e, int d, i64 G) {
   if (d == 0 || X[L] == 0u) return;
   const i64 sh = G + 31 - e;
   if (sh >= 32 * (L + 1)) return;
@@ -584,59 +584,93 @@ __device__ __forceinline__ void cauchy_term(u32 (&acc)[L + 3], u32 (&X)[L + 1], 
 }
 
 // One Cauchy-step entry: h + sum_i x_i * 2 / (D - D_i) for column n of a group whose prime poles are the
-// slots colg[i] (i < nc) at doubled positions coldg[i], h the h_inf entry hidx of hm/he, with one rounding.
-// E = largest exponent of the nonzero x_i (slot buffer rm/re, plane stride rps, N columns per slot);
+// slots colg[i] (i < nc) at doubled positions coldg[i], h the h_inf entry (hrow, n) of hm/he, with one
+// rounding.  E = largest exponent of the nonzero x_i (slot buffer rm/re, plane stride rps, N columns per slot);
 // grid G = max(E - 31, e_h - 32) (either side alone when the other is absent); term i =
 // sign(d) floor(m_i 2^32 / |d| / 2^(G + 31 - e_i)) (signed mantissa, floor toward -infinity; none when the
-// shift reaches 32 (L + 1)) and h truncated onto 2^G; exact sum in L + 3 limbs, one rounding.  Returns the
+// shift reaches 32 (L + 1)) and h truncated onto 2^G; exact sum in L + 2 limbs (every term and h are
+// below 2^(32 L + 30) in magnitude, so fewer than 2^32 of them cannot overflow), one rounding.  Returns the
 // exponent, mantissa in r.
-template <int L>
+// UNI: the group (colg, coldg, nc, D, hrow) is the same for every lane of the wave.  Then the column list
+// is read through the scalar cache and each slot's plane base colg[i] * N is a scalar, so a term costs the
+// limb loads alone (lane offset n * 4); otherwise every index is a per-lane vector load.
+template <int L, bool UNI = false>
 __device__ __forceinline__ i64 cauchy_entry(const int32_t* __restrict__ rm, const i64* __restrict__ re, i64 rps,
                                             i64 N, i64 n, const int64_t* __restrict__ colg,
                                             const int32_t* __restrict__ coldg, int nc, int D,
                                             const int32_t* __restrict__ hm, const i64* __restrict__ he, i64 hps,
-                                            i64 hidx, u32 (&r)[L]) {
-  i64 E = B3D_EZ;
-  for (int i = 0; i < nc; ++i) {
-    const i64 idx = colg[i] * N + n;
-    if (load_limb(rm, rps, L - 1, idx) != 0u) {
-      const i64 ei = re[idx];
-      E = ei > E ? ei : E;
+                                            i64 hrow, u32 (&r)[L]) {
+  // slot i's base (UNI: a scalar offset, lane index n) or its entry (lane index 0)
+  auto at = [&](i64 slot, i64& base, i64& lane) {
+    if constexpr (UNI) {
+      base = slot * N;
+      lane = n;
+    } else {
+      base = 0;
+      lane = slot * N + n;
     }
+  };
+  // zero entries carry the exponent EZ (the canonical zero every kernel writes), below that of any nonzero
+  // entry, so E is the plain maximum of the exponents: independent loads, no limb read
+  i64 E = B3D_EZ;
+#pragma unroll 4
+  for (int i = 0; i < nc; ++i) {
+    i64 b, l;
+    at(colg[i], b, l);
+    const i64 ei = re[b + l];
+    E = ei > E ? ei : E;
   }
   // only the top limb of h here: its other limbs are loaded after the terms, so they hold no registers
   // through the loop
-  const bool hz = load_limb(hm, hps, L - 1, hidx) == 0u;
-  const i64 eh = he[hidx];
+  i64 hb, hl;
+  at(hrow, hb, hl);
+  const bool hz = load_limb(hm + hb, hps, L - 1, hl) == 0u;
+  const i64 eh = he[hb + hl];
   if (E == B3D_EZ && hz) {
     zero(r);
     return B3D_EZ;
   }
   const i64 G = E == B3D_EZ ? eh - 32 : (hz ? E - 31 : ((E - 31) > (eh - 32) ? (E - 31) : (eh - 32)));
-  u32 acc[L + 3];
+  u32 acc[L + 2];
   zero(acc);
   if (E != B3D_EZ) {
+    // UNI: the next column's slot and position are read one term ahead (scalar registers)
+    i64 cn = colg[0];
+    int dn = coldg[0];
     for (int i = 0; i < nc; ++i) {
-      const int d = D - coldg[i];
+      i64 c;
+      int d;
+      if constexpr (UNI) {
+        c = cn;
+        d = D - dn;
+        if (i + 1 < nc) {
+          cn = colg[i + 1];
+          dn = coldg[i + 1];
+        }
+      } else {
+        c = colg[i];
+        d = D - coldg[i];
+      }
       if (d == 0) continue;
-      const i64 idx = colg[i] * N + n;
+      i64 b, l;
+      at(c, b, l);
       u32 X[L + 1];   // m 2^32
       X[0] = 0u;
-      load_desc<L>(X, rm, rps, idx);
-      cauchy_term<L>(acc, X, re[idx], d, G);
+      load_desc<L>(X, rm + b, rps, l);
+      cauchy_term<L>(acc, X, re[b + l], d, G);
     }
   }
   if (!hz) {
     // |m_h| 2^(e_h - G) truncated = floor(|m_h| 2^32 / 2^(32 - e_h + G)), e_h - G <= 32
     u32 y[L + 1];
     y[0] = 0u;
-    load_desc<L>(y, hm, hps, hidx);
+    load_desc<L>(y, hm + hb, hps, hl);
     const u32 sh_ = to_mag(y);
     const i64 s = 32 - (eh - G);
     sar(y, s > 32 * (L + 1) ? 32 * (L + 1) : (int)s, 0u);
     cadd(acc, y, sh_);
   }
-  return norm<L, L + 3>(acc, G, r);
+  return norm<L, L + 2>(acc, G, r);
 }
 
 // One windowed moment: sum_w y_w n_w^k for the staged entries y_w = stage[w, row, c] (w < W), n_w = n0 + w,

@@ -81,9 +81,10 @@ template <int L> static int64_t shr_t(const uint32_t* x, int s, int left, uint32
 template <int L>
 static int64_t cauchy_t(const int32_t* rm, const int64_t* re, int64_t rps, int64_t N, int64_t n, const int64_t* colg,
                         const int32_t* coldg, int nc, int D, const int32_t* hm, const int64_t* he, int64_t hps,
-                        int64_t hidx, uint32_t* r) {
+                        int64_t hrow, int uni, uint32_t* r) {
   u32 R[L];
-  const int64_t e = mp::cauchy_entry<L>(rm, re, rps, N, n, colg, coldg, nc, D, hm, he, hps, hidx, R);
+  const int64_t e = uni ? mp::cauchy_entry<L, true>(rm, re, rps, N, n, colg, coldg, nc, D, hm, he, hps, hrow, R)
+                        : mp::cauchy_entry<L, false>(rm, re, rps, N, n, colg, coldg, nc, D, hm, he, hps, hrow, R);
   copy_out(r, R);
   return e;
 }
@@ -128,9 +129,9 @@ int64_t hc_norm(int L, const uint32_t* w, int LW, int64_t e, uint32_t* r) { DISP
 int64_t hc_div(int L, const uint32_t* x, uint32_t d, uint32_t* q) { DISP(L, divs_t<LL>(x, d, q)) }
 int64_t hc_shift(int L, const uint32_t* x, int s, int left, uint32_t* out) { DISP(L, shr_t<LL>(x, s, left, out)) }
 int64_t hc_cauchy(int L, const int32_t* rm, const int64_t* re, int64_t rps, int64_t N, int64_t n, const int64_t* colg,
-                  const int32_t* coldg, int nc, int D, const int32_t* hm, const int64_t* he, int64_t hps, int64_t hidx,
-                  uint32_t* r) {
-  DISP(L, cauchy_t<LL>(rm, re, rps, N, n, colg, coldg, nc, D, hm, he, hps, hidx, r))
+                  const int32_t* coldg, int nc, int D, const int32_t* hm, const int64_t* he, int64_t hps, int64_t hrow,
+                  int uni, uint32_t* r) {
+  DISP(L, cauchy_t<LL>(rm, re, rps, N, n, colg, coldg, nc, D, hm, he, hps, hrow, uni, r))
 }
 int64_t hc_dot(int L, const int32_t* am, int64_t aps, int64_t abase, int64_t ea, const int32_t* xm, const int64_t* xe,
                int64_t xps, int64_t xbase, int64_t xstride, int S, uint32_t* r) {

@@ -3,6 +3,8 @@ initial data with several right-hand sides (reference tests/test_blocks_recursio
   * bit for bit against oracle.recursion_ref.recursion_bits (the kernels' exact rounding semantics);
   * within 2^-(32L - 64) of the exact rational recursion relative to each (pole, spin) block;
   * with slot recycling (reuse=True) every residue, captured right after its level, is identical.
+Rows whose Smax^2 B columns fill whole waves to at least 3/4 take the row-uniform Cauchy kernel (cases with 63
+and 64 columns), the others the general one.
 """
 import random
 from fractions import Fraction as F
@@ -17,9 +19,14 @@ from oracle import structure_ref as OS
 pytestmark = pytest.mark.gpu
 
 CASES = [dict(tj12=4, tj43=2, tjmax=6, order=7, L=10, B=3, seed=1, pp=((0, 1), (1, 0))),
-         dict(tj12=8, tj43=8, tjmax=4, order=6, L=10, B=2, seed=2, pp=((0, 1), (1, 0))),
+         # Smax 3, B 7: rows of 63 columns on 64 lanes (one idle lane per wave), the row-uniform Cauchy kernel
+         dict(tj12=4, tj43=2, tjmax=6, order=7, L=10, B=7, seed=6, pp=((0, 1), (1, 0))),
+         # Smax 5, B 3: 75 columns per row, waves inside one row and waves across two in the general kernel
+         dict(tj12=8, tj43=8, tjmax=4, order=6, L=10, B=3, seed=2, pp=((0, 1), (1, 0))),
          dict(tj12=3, tj43=1, tjmax=5, order=6, L=6, B=2, seed=3, pp=((0, 1), (1, 0))),
-         dict(tj12=2, tj43=2, tjmax=6, order=8, L=34, B=2, seed=4, pp=((0, 0), (1, 1)))]
+         dict(tj12=2, tj43=2, tjmax=6, order=8, L=34, B=2, seed=4, pp=((0, 0), (1, 1))),
+         # Smax 4, B 4: 64 columns per row, the row-uniform Cauchy kernel
+         dict(tj12=6, tj43=2, tjmax=4, order=5, L=34, B=4, seed=5, pp=((0, 1), (1, 0)))]
 
 
 def random_operator(plan, tj12, tj43, pp, order, B, seed):

@@ -147,18 +147,20 @@ def test_cauchy_entry(lib, L):
         coldg = np.array([rng.randint(-16000, 16000) if big else rng.randint(-300, 300) for _ in range(nc)], np.int32)
         D = rng.randint(-16000, 16000) if big else (int(coldg[rng.randrange(nc)]) if rng.random() < 0.5
                                                    else rng.randint(-300, 300))
-        hv = [rp(rng, L, spread=rng.choice([3, 300])) for _ in range(2)]
+        hv = [rp(rng, L, spread=rng.choice([3, 300])) for _ in range(2 * N)]     # h_inf rows 0 and 1
+        n = rng.randrange(N)
         if trial % 5 == 0:
-            hv[1] = (0, R.EZ)
+            hv[N + n] = (0, R.EZ)
         hm = np.stack([limbs(v[0], L) for v in hv], 1).view(np.int32).copy()
         he = np.array([v[1] for v in hv], np.int64)
-        n = rng.randrange(N)
-        r = np.zeros(L, np.uint32)
-        e = lib.hc_cauchy(L, ptr(rm), ptr(re_), I64(nslot * N), I64(N), I64(n), ptr(colg), ptr(coldg), nc, D,
-                          ptr(hm), ptr(he), I64(2), I64(1), ptr(r))
         xs = [vals[int(c) * N + n] for c in colg]
         ds = [D - int(c) for c in coldg]
-        assert got(r, e) == R.cauchy(xs, ds, L, hv[1]), trial
+        want = R.cauchy(xs, ds, L, hv[N + n])
+        for uni in (0, 1):                                  # per-lane indices and wave-uniform group
+            r = np.zeros(L, np.uint32)
+            e = lib.hc_cauchy(L, ptr(rm), ptr(re_), I64(nslot * N), I64(N), I64(n), ptr(colg), ptr(coldg), nc, D,
+                              ptr(hm), ptr(he), I64(2 * N), I64(1), uni, ptr(r))
+            assert got(r, e) == want, (trial, uni)
 
 
 @pytest.mark.parametrize("L", [2, 4, 10, 34, 50])
