@@ -143,6 +143,37 @@ const char* b3d_last_error(void) { return b3d::g_err.c_str(); }
 
 int b3d_abi_version(void) { return B3D_ABI_VERSION; }
 
+int b3d_clear_hip_error(void) { return (int)hipGetLastError(); }
+
+int b3d_end_capture(void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &st) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  if (st == hipStreamCaptureStatusNone) return 0;
+  hipGraph_t g = nullptr;
+  (void)hipStreamEndCapture(s, &g);
+  if (g) (void)hipGraphDestroy(g);
+  (void)hipGetLastError();
+  return 1;
+}
+
+int b3d_thread_capture_mode(int mode) {
+  if (mode < 0 || mode > 2) {
+    b3d::set_error("b3d_thread_capture_mode: mode must be 0 (global), 1 (thread-local) or 2 (relaxed)");
+    return -1;
+  }
+  hipStreamCaptureMode m = (hipStreamCaptureMode)mode;
+  const hipError_t st = hipThreadExchangeStreamCaptureMode(&m);
+  if (st != hipSuccess) {
+    b3d::set_error(std::string("b3d_thread_capture_mode: ") + hipGetErrorString(st));
+    return -1;
+  }
+  return (int)m;
+}
+
 int b3d_supported_limbs(int L) {
   switch (L) {
 #define B3D_SUP(x) case x:

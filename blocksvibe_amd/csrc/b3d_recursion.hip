@@ -133,8 +133,9 @@ int b3d_cauchy_level(int L, const int32_t* rm, const int64_t* re, int64_t rps, i
 int b3d_pole_level(int L, int S, int64_t B, int64_t T, const int32_t* Hm, const int64_t* He, int64_t hps,
                    const int64_t* thrw, const int32_t* tR, const int32_t* tQL, const int64_t* tslot,
                    const int32_t* R28, const uint32_t* Rsg, const int64_t* Re, int64_t rtps, const int32_t* Q28,
-                   const uint32_t* Qsg, const int64_t* Qe, int64_t qps, int32_t* Ym, int64_t* Ye, int64_t yps,
-                   int32_t* rm, int64_t* re, int64_t rps, void* stream) {
+                   const uint32_t* Qsg, const int64_t* Qe, int64_t qps, const int8_t* Rd8, const int32_t* Rcorr,
+                   const int8_t* Qd8, const int32_t* Qcorr, int32_t* Ym, int64_t* Ye, int64_t yps, int32_t* rm,
+                   int64_t* re, int64_t rps, void* stream) {
   if (T <= 0 || B <= 0) return 0;
   if (S < 1 || S > 16) {
     b3d::set_error("b3d_pole_level: structure count S must be 1..16");
@@ -142,6 +143,11 @@ int b3d_pole_level(int L, int S, int64_t B, int64_t T, const int32_t* Hm, const 
   }
   if (b3d::check_planes("b3d_pole_level", {hps, rtps, qps, yps, rps})) return 2;
   hipStream_t s = (hipStream_t)stream;
+  if (Rd8 && Rcorr && Qd8 && Qcorr && (int64_t)S * B >= 32) {
+    // at least 32 rows share each constant row: int8 MFMA products
+    if (b3d::pole_mma(L, Hm, He, hps, thrw, tR, Rd8, Rcorr, Re, nullptr, T, S, B, Ym, Ye, yps, s)) return 1;
+    return b3d::pole_mma(L, Ym, Ye, yps, nullptr, tQL, Qd8, Qcorr, Qe, tslot, T, S, B, rm, re, rps, s);
+  }
   const int64_t n = T * S * S * B;
   B3D_DISPATCH_L(L, k_pole_r<LL><<<dim3(grid_for(n, 256)), dim3(256), 0, s>>>(Hm, He, hps, thrw, tR, T, S, B, R28, Rsg, Re, rtps, Ym, Ye, yps));
   if (b3d::check_launch("b3d_pole_level (R)")) return 1;

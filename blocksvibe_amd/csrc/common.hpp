@@ -22,6 +22,14 @@ inline int check_planes(const char* what, std::initializer_list<int64_t> strides
     }
   return 0;
 }
+
+// The pole step's dot products on int8 MFMA (b3d_pole.hip): X entry (xr, u, j, k) at
+// Xm[xr S SB + u SB + j B + k] (xr = xrow[task], or task when xrow is null), output (task, s0, n = u B + k) at
+// Om[orow S SB + s0 SB + n], constant row ctab[task] S + s0 with the digit and correction tables of
+// b3d_pole_tables and row exponents Ce.
+int pole_mma(int L, const int32_t* Xm, const int64_t* Xe, int64_t xps, const int64_t* xrow, const int32_t* ctab,
+             const int8_t* d8, const int32_t* corr, const int64_t* Ce, const int64_t* orow, int64_t T, int S, int64_t B,
+             int32_t* Om, int64_t* Oe, int64_t ops, hipStream_t s);
 }  // namespace b3d
 
 // Limb counts with compiled kernels.  L = 2 ceil(precision / 64) + 2 (precision rounded up to whole

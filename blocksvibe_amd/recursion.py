@@ -43,6 +43,17 @@ def _split28(m, L):
     return o28, sg
 
 
+def _pole_tables(m, L, S):
+    """row-aligned mantissas (L limb planes of (rows, S, S)) -> (balanced base-256 digits, column
+    corrections) for the pole step's int8 MFMA products, on the GPU"""
+    n = m[0].numel()
+    ds = _lib.lib().b3d_pole_digits_bytes(L)
+    d8 = torch.empty((n, ds), dtype=torch.int8, device=m.device)
+    corr = torch.empty((n // S, 8 * L), dtype=torch.int32, device=m.device)
+    _lib.call("b3d_pole_tables", L, S, m.data_ptr(), n, n // S, d8.data_ptr(), corr.data_ptr(), mpf.stream_ptr())
+    return d8, corr
+
+
 class Recursion:
     """Device constants and level tables of one Plan at L limbs.
     M: dict {Mkey: matrix (rows of numbers)} for every key of plan.Mkeys (absent keys: zero matrices), an
@@ -94,6 +105,10 @@ class Recursion:
         r._constants()
         return r
 
+    # pole-step products on int8 MFMA where at least 32 rows share a constant row (bit-identical to the VALU
+    # path; off by default while its kernel is slower than the VALU one at the benchmark's shapes)
+    mma = False
+
     def _constants(self):
         plan, Sm, L = self.plan, self.plan.Smax, self.L
         self.Ralign_m, self.Ralign_e = mpf.align_rows(self.Mtab)            # (L, nM, S, S), (nM, S)
@@ -107,6 +122,8 @@ class Recursion:
         # the pole step reads both constant operands as 28-bit magnitude limbs + sign masks
         self.R28, self.Rsg = _split28(self.Ralign_m, L)
         self.QL28, self.QLsg = _split28(self.QLalign_m, L)
+        self.Rd8, self.Rcorr = _pole_tables(self.Ralign_m, L, Sm)
+        self.QLd8, self.QLcorr = _pole_tables(self.QLalign_m, L, Sm)
 
     def _upload_levels(self):
         plan, d = self.plan, mpf.dev()
@@ -176,6 +193,8 @@ class Recursion:
             _lib.call("b3d_pole_level", L, Sm, B, lv["T"], P(H.m), P(H.e), H.numel(), P(lv["t_hrow"]),
                       P(lv["t_R"]), P(lv["t_qL"]), P(lv["t_slot"]), P(self.R28), P(self.Rsg), P(self.Ralign_e),
                       self.R28[0].numel(), P(self.QL28), P(self.QLsg), P(self.QLalign_e), self.QL28[0].numel(),
+                      P(self.Rd8) if self.mma else 0, P(self.Rcorr) if self.mma else 0,
+                      P(self.QLd8) if self.mma else 0, P(self.QLcorr) if self.mma else 0,
                       P(Y.m), P(Y.e), Y.numel(), P(res.m), P(res.e), res.numel(), st)
             if prof is not None:
                 torch.cuda.synchronize()

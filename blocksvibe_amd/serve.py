@@ -9,7 +9,11 @@ next batch as soon as a worker and memory are free, and batches on different str
 device.  Plans, uploaded level tables, operator constants (M, qL, R tables) and write_output maps are cached
 on the GPU across requests.  A batch shape (operator, sign, batch size, coordinates) that has run once is
 captured as a HIP graph the next time and replayed from then on (static inputs refilled by device copies),
-within a memory budget for the graphs' private pools.
+within a memory budget for the graphs' private pools.  The HIP runtime judges a potentially unsafe call (a
+synchronous copy, a device-wide synchronisation) by the calling thread's capture mode against every capture in
+the process, so the worker threads and the threads that submit requests are switched to the relaxed mode, a
+capture runs alone (the other workers' device work waits for it) on a fresh stream in the thread-local mode,
+and a capture that fails anyway is ended and leaves its batch shape to eager runs.
 
 Every kernel of the chain treats each right-hand-side column independently (per-column alignment, one
 rounding per entry, A operands that do not depend on the batch), so a request's result is bit-identical
@@ -19,12 +23,14 @@ Analogue of the reference's spool batching of concurrent invocations and per-GPU
 blocksvibe/cli.py:238-265, 312-321, 569-637), in one process.
 """
 import collections
+import contextlib
+import sys
 import threading
 import time
 
 import torch
 
-from . import mpf
+from . import _lib, mpf
 from .derivatives import Derivs
 from .output import Output
 from .plan import Plan
@@ -86,17 +92,89 @@ class _Operator:
         return pl.Smax ** 2 * (4 * L + 8) * 2 * (pl.order + 1) * pl.Jh
 
 
+_MODE_THREAD_LOCAL, _MODE_RELAXED = 1, 2     # hipStreamCaptureMode
+_tls = threading.local()
+
+
+def _capture_mode(mode):
+    """set the calling thread's stream-capture mode, return the previous one"""
+    prev = _lib.lib().b3d_thread_capture_mode(mode)
+    if prev < 0:
+        raise _lib.B3DError(_lib.lib().b3d_last_error().decode())
+    return prev
+
+
+def _relax_thread():
+    """relaxed capture checking for the calling thread (once per thread): its synchronous copies and
+    synchronisations then leave the captures of the worker threads alone"""
+    if not getattr(_tls, "relaxed", False):
+        _capture_mode(_MODE_RELAXED)
+        _tls.relaxed = True
+
+
+class _Gate:
+    """many eager sections or one capture at a time (a capture waits for the running eager sections and
+    holds new ones back)"""
+
+    def __init__(self):
+        self.c = threading.Condition()
+        self.active, self.capturing, self.waiting = 0, False, 0
+
+    @contextlib.contextmanager
+    def eager(self):
+        with self.c:
+            while self.capturing or self.waiting:
+                self.c.wait()
+            self.active += 1
+        try:
+            yield
+        finally:
+            with self.c:
+                self.active -= 1
+                self.c.notify_all()
+
+    @contextlib.contextmanager
+    def capture(self):
+        with self.c:
+            self.waiting += 1
+            while self.capturing or self.active:
+                self.c.wait()
+            self.waiting -= 1
+            self.capturing = True
+        try:
+            yield
+        finally:
+            with self.c:
+                self.capturing = False
+                self.c.notify_all()
+
+
 class _Graph:
     """one captured request pipeline (operator, sign, batch size, coordinates): static h_inf inputs, static
     outputs; replays are serialised by a lock, and results are copied out before the next replay"""
 
     def __init__(self, fn, hs):
         self.hin = [mpf.MPF(h.m.clone(), h.e.clone()) for h in hs]
-        torch.cuda.current_stream().synchronize()
+        work = torch.cuda.current_stream()
+        work.synchronize()
+        cap = torch.cuda.Stream()
         self.graph = torch.cuda.CUDAGraph()
         before = torch.cuda.memory_allocated()
-        with torch.cuda.graph(self.graph, capture_error_mode="thread_local"):
-            self.out = fn(self.hin)
+        # the thread-local mode makes an unsafe call of this thread during the capture an error rather than
+        # an eagerly executed step missing from the graph
+        prev = _capture_mode(_MODE_THREAD_LOCAL)
+        try:
+            with torch.cuda.graph(self.graph, stream=cap, capture_error_mode="thread_local"):
+                self.out = fn(self.hin)
+        except BaseException:
+            # a failed capture can leave the capture open and the capture stream current: end it, go back
+            _lib.lib().b3d_end_capture(cap.cuda_stream)
+            _lib.lib().b3d_clear_hip_error()
+            torch.cuda.set_stream(work)
+            raise
+        finally:
+            _capture_mode(prev)
+        cap.synchronize()
         self.bytes = max(0, torch.cuda.memory_allocated() - before)
         self.lock = threading.Lock()
 
@@ -128,6 +206,8 @@ class Server:
         self.use_graphs = graphs
         self.graph_budget = graph_budget if graph_budget is not None else self.budget // 4
         self.graphs, self.seen, self.graph_bytes, self.replays = {}, set(), 0, 0
+        self.capture_failures = 0
+        self.gate = _Gate()
         self.graph_lock = threading.Lock()
         self.streams = [torch.cuda.Stream() for _ in range(streams)]
         self.workers = [threading.Thread(target=self._work, args=(s,), daemon=True) for s in self.streams]
@@ -136,6 +216,7 @@ class Server:
 
     # ------------------------------------------------------------------------------------------ API
     def submit(self, req):
+        _relax_thread()
         req.t_arrival = time.time()
         with self.lock:
             self.queue.append(req)
@@ -177,12 +258,14 @@ class Server:
                 if op is None:                              # first request of an operator: build it unlocked
                     self.lock.release()
                     try:
-                        self._operator(head.key)
+                        with self.gate.eager():
+                            self._operator(head.key)
+                            torch.cuda.current_stream().synchronize()
                     finally:
                         self.lock.acquire()
                     continue
                 p0 = (1 - head.sign) // 2
-                with self.setup_lock:
+                with self.gate.eager(), self.setup_lock:    # may build derivative constants
                     per = op.per_rhs_bytes(p0) + op.h_bytes() * len(head.hs)
                 room = self.budget - self.in_use
                 if per <= room or self.in_use == 0:
@@ -202,6 +285,7 @@ class Server:
             self.lock.wait(0.05)
 
     def _work(self, stream):
+        _relax_thread()
         while True:
             with self.lock:
                 batch, cost = self._take_batch()
@@ -209,8 +293,7 @@ class Server:
                 return
             try:
                 with torch.cuda.stream(stream):
-                    self._run(batch)
-                    stream.synchronize()
+                    self._run(batch)                # returns with the stream synchronised
                 t = time.time()
                 for r in batch:
                     r.t_done = t
@@ -248,31 +331,44 @@ class Server:
             r.t_start = t0
         coords = tuple(sorted({(c, r.d12) for r in batch for c in r.coords}))
         nh = len(head.hs)
-        hs = [mpf.cat([r.hs[i] for r in batch], 4) if len(batch) > 1 else head.hs[i] for i in range(nh)]
+        inputs = lambda: [mpf.cat([r.hs[i] for r in batch], 4) if len(batch) > 1 else head.hs[i] for i in range(nh)]
         gkey = (head.key, head.sign, len(batch), nh, coords)
         fn = lambda x: self._pipeline(op, head.key, head.sign, coords, x)
         graph, capture = None, False
         if self.use_graphs:
             with self.graph_lock:
-                graph = self.graphs.get(gkey)
+                graph = self.graphs.get(gkey) or None          # None: claimed or failed, run eagerly
                 capture = gkey not in self.graphs and gkey in self.seen and self.graph_bytes < self.graph_budget
                 if capture:
                     self.graphs[gkey] = None            # claimed: other workers run this shape eagerly meanwhile
                 self.seen.add(gkey)
         if capture:
             # no lock is held while capturing: the pipeline takes the setup lock for cached constants
-            graph = _Graph(fn, hs)
+            try:
+                with self.gate.capture():
+                    graph = _Graph(fn, inputs())
+            except Exception as e:                          # the shape stays eager
+                graph = None
+                msg = str(e).splitlines()[0] if str(e) else type(e).__name__
+                print(f"serve: capture of {gkey[:2]} x{len(batch)} failed, running it eagerly ({msg})",
+                      file=sys.stderr, flush=True)
             with self.graph_lock:
-                self.graphs[gkey] = graph
-                self.graph_bytes += graph.bytes
-        if graph is not None:
-            with graph.lock:
-                outs = graph.replay(hs)
-                self.replays += 1
-                self._deliver(batch, outs, copy=True)
+                self.graphs[gkey] = graph if graph is not None else False
+                if graph is not None:
+                    self.graph_bytes += graph.bytes
+                else:
+                    self.capture_failures += 1
+        with self.gate.eager():
+            hs = inputs()
+            if graph is not None:
+                with graph.lock:
+                    outs = graph.replay(hs)
+                    self.replays += 1
+                    self._deliver(batch, outs, copy=True)
+                    torch.cuda.current_stream().synchronize()
+            else:
+                self._deliver(batch, fn(hs), copy=False)
                 torch.cuda.current_stream().synchronize()
-        else:
-            self._deliver(batch, fn(hs), copy=False)
 
     @staticmethod
     def _deliver(batch, outs, copy):

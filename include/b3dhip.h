@@ -19,11 +19,24 @@
 extern "C" {
 #endif
 
-#define B3D_ABI_VERSION 5
+#define B3D_ABI_VERSION 6
 
 /* Last error message of the calling thread (empty if none). */
 const char* b3d_last_error(void);
 int b3d_abi_version(void);
+/* Set the calling thread's stream-capture mode (0 global, 1 thread-local, 2 relaxed; HIP's
+ * hipStreamCaptureMode) and return the previous one, -1 on error.  The HIP runtime judges a potentially
+ * unsafe call (a synchronous copy, a device-wide synchronisation) by the calling thread's mode against every
+ * capture in the process, so a thread left in the global mode invalidates captures running in other threads;
+ * the serving loop relaxes its worker and submitting threads and captures in the thread-local mode.  No
+ * counterpart in the reference (it has no captured pipelines). */
+int b3d_thread_capture_mode(int mode);
+/* Read and clear the calling thread's pending HIP error (hipGetLastError), e.g. after a failed capture, so
+ * that it is not reported by the next launch; returns the hipError_t code (0: none). */
+int b3d_clear_hip_error(void);
+/* End a capture still open on `stream` (one that failed part way) and discard its graph, so that the stream
+ * can be used again; returns 1 if a capture was ended, 0 if none was open. */
+int b3d_end_capture(void* stream);
 /* 1 if kernels are compiled for L limbs (reference cli.py:270-274 gives L from the precision). */
 int b3d_supported_limbs(int L);
 
@@ -72,13 +85,27 @@ int b3d_cauchy_level(int L, const int32_t* rm, const int64_t* re, int64_t rps, i
 /* Pole step for T tasks: Y = Ralign[tR[t]] . H[thrw[t]]^T, then res[tslot[t]] = QLalign[tQL[t]] . Y^T,
  * S x S structure matrices, B right-hand sides.  Ralign/QLalign are given in the form b3d_mpf_to28 makes of
  * the row-aligned mantissas: 28-bit magnitude limb planes R28/Q28 (b3d_limbs28(L) planes of (ntab, S, S),
- * plane strides rtps/qps) and sign masks Rsg/Qsg (ntab, S, S), with row exponents Re/Qe (ntab, S).
+ * plane strides rtps/qps) and sign masks Rsg/Qsg (ntab, S, S), with row exponents Re/Qe (ntab, S); and in
+ * the form b3d_pole_tables makes (digits Rd8/Qd8, corrections Rcorr/Qcorr; may be null).  With the digit
+ * tables and S*B >= 32 the products run on int8 MFMA, otherwise on the VALU; both give the same bits.
  * Y: scratch (L, T, S, S, B), plane stride yps. */
 int b3d_pole_level(int L, int S, int64_t B, int64_t T, const int32_t* Hm, const int64_t* He, int64_t hps,
                    const int64_t* thrw, const int32_t* tR, const int32_t* tQL, const int64_t* tslot,
                    const int32_t* R28, const uint32_t* Rsg, const int64_t* Re, int64_t rtps, const int32_t* Q28,
-                   const uint32_t* Qsg, const int64_t* Qe, int64_t qps, int32_t* Ym, int64_t* Ye, int64_t yps,
-                   int32_t* rm, int64_t* re, int64_t rps, void* stream);
+                   const uint32_t* Qsg, const int64_t* Qe, int64_t qps, const int8_t* Rd8, const int32_t* Rcorr,
+                   const int8_t* Qd8, const int32_t* Qcorr, int32_t* Ym, int64_t* Ye, int64_t yps, int32_t* rm,
+                   int64_t* re, int64_t rps, void* stream);
+
+/* Bytes per constant entry in the digit tables of b3d_pole_tables (4L + 4), or -1 for an unsupported L. */
+int b3d_pole_digits_bytes(int L);
+
+/* Constant operands of the pole step for its int8 MFMA path: the nrows*S row-aligned entries m (two's
+ * complement, plane stride ps; entry j of row cr at cr*S + j) as balanced base-256 digits d8
+ * (b3d_pole_digits_bytes(L) bytes per entry: 4L + 1 digits in [-128, 127], then zeros) and per row the
+ * column corrections corr (nrows, 8L) int32, corr[cr][c] = 128 sum_j sum_{i <= 4L-2} d_j[c - i].  No
+ * counterpart in the reference (its smallgemm multiplies limb by limb on every call). */
+int b3d_pole_tables(int L, int S, const int32_t* m, int64_t ps, int64_t nrows, int8_t* d8, int32_t* corr,
+                    void* stream);
 
 /* ---- derivative moments during the recursion (replace the momacc/momflush kernels and the staging of
  *      reference blocksvibe/derivatives.py:46-189, 474-510) ---- */

@@ -57,3 +57,36 @@ def test_graph_replays_match_eager():
     for mk, res in got[2:]:
         assert _same(res, want2 if mk is second else want1)
     print(f"replays {srv.replays}, graph pools {srv.graph_bytes / 2**20:.1f} MiB")
+
+
+def test_capture_beside_synchronous_copies():
+    """Captures in a worker thread while the submitting thread makes synchronous host-device copies and
+    another worker runs eagerly: no capture is invalidated, and the replays equal the eager results."""
+    import threading
+    from blocksvibe_amd.plan import Plan
+    from blocksvibe_amd.serve import Server
+    L, pp = 10, ((0, 0), (1, 1))
+    plan = Plan(4, 2, 6, 10, pp)
+    key = (4, 2, 6, 10, 5, 8, L, pp, 0)
+    other_plan = Plan(2, 2, 6, 8, pp)
+    other = (2, 2, 6, 8, 5, 6, L, pp, 1)
+    first, noise = _req(key, 1, plan, L, 400), _req(other, -1, other_plan, L, 500)
+
+    eager = Server(streams=1, graphs=False)
+    want = _host(eager.submit(first()).wait(300))
+    eager.close()
+
+    srv = Server(streams=2, graphs=True)
+    _host(srv.submit(first()).wait(300))                 # eager: the shape is seen
+    got = []
+    for _ in range(3):
+        r = srv.submit(first())                          # captured the first time, then replayed
+        n = srv.submit(noise())
+        while not r.done.is_set():                       # synchronous copies in this thread meanwhile
+            torch.tensor([1, 2, 3], device="cuda").cpu()
+        got.append(_host(r.wait(300)))
+        n.wait(300)
+    srv.close()
+    assert srv.capture_failures == 0, srv.capture_failures
+    assert srv.replays >= 2, srv.replays
+    assert all(_same(g, want) for g in got)

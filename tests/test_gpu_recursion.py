@@ -4,7 +4,7 @@ initial data with several right-hand sides (reference tests/test_blocks_recursio
   * within 2^-(32L - 64) of the exact rational recursion relative to each (pole, spin) block;
   * with slot recycling (reuse=True) every residue, captured right after its level, is identical.
 Rows whose Smax^2 B columns fill whole waves to at least 3/4 take the row-uniform Cauchy kernel (cases with 63
-and 64 columns), the others the general one.
+and 64 columns), the others the general one; with Smax B >= 32 the pole step runs on int8 MFMA (Smax 5, B 7).
 """
 import random
 from fractions import Fraction as F
@@ -25,6 +25,8 @@ CASES = [dict(tj12=4, tj43=2, tjmax=6, order=7, L=10, B=3, seed=1, pp=((0, 1), (
          dict(tj12=8, tj43=8, tjmax=4, order=6, L=10, B=3, seed=2, pp=((0, 1), (1, 0))),
          dict(tj12=3, tj43=1, tjmax=5, order=6, L=6, B=2, seed=3, pp=((0, 1), (1, 0))),
          dict(tj12=2, tj43=2, tjmax=6, order=8, L=34, B=2, seed=4, pp=((0, 0), (1, 1))),
+         # Smax 5, B 7: 35 rows per constant row, the pole step on int8 MFMA
+         dict(tj12=8, tj43=8, tjmax=4, order=6, L=10, B=7, seed=7, pp=((0, 1), (1, 0))),
          # Smax 4, B 4: 64 columns per row, the row-uniform Cauchy kernel
          dict(tj12=6, tj43=2, tjmax=4, order=5, L=34, B=4, seed=5, pp=((0, 1), (1, 0)))]
 
@@ -74,6 +76,7 @@ def test_recursion_bits_and_exact(case):
     print(f"case {c}: {sum(l.T for l in plan.levels if l)} tasks, Smax {plan.Smax}", flush=True)
     h = h_tensor(H, plan, L)
     rec = Recursion(plan, M, L)
+    rec.mma = True                       # the int8 MFMA pole step where Smax B >= 32, the VALU one elsewhere
     res = rec.run(h)
     got = res.to_host()
     # slot reuse: capture the slots written at each level right after it
