@@ -14,13 +14,12 @@
 // same 2L + 2 limb integer as the VALU path, so the rounding is identical.  The accumulators start from the
 // corrections.
 //
-// Work item = (constant group (task, s0), block of 64 rows): the rows share the constant row, so one
-// workgroup stages the rows' aligned bytes in LDS (A operand, 16-byte chunks; up to six operands at a
-// time, in the space the column sums take afterwards), the constant digits
-// reversed in four byte-shifted copies (so that every B fragment d_j[c - i0 - 0..15] is four aligned LDS
-// dwords), and runs v_mfma_i32_16x16x64_i8 over 4 row tiles x (8L / 16) column tiles (column tiles dealt
-// round robin to the four waves).  Column sums go through LDS to a carry pass (four segments per row in
-// parallel) and a per-row assembly + norm.
+// Work item = (task, block of 64 rows), one workgroup of 8 waves: the rows' aligned bytes (A operand,
+// 16-byte chunks) are staged in LDS once for the task's S constant rows, with every constant row's digits
+// and corrections; for each constant row, the math waves build its digits reversed in four byte-shifted
+// copies (so that every B fragment d_j[c - i0 - 0..15] is four aligned LDS dwords) and run
+// v_mfma_i32_16x16x64_i8 over 4 row tiles x (8L / 16) column tiles (dealt round robin to waves 0-3), while
+// waves 4-7 carry the previous constant row's column sums (four segments per row) and round them.
 #include "common.hpp"
 #include "mp.hpp"
 #include "../../include/b3dhip.h"
@@ -32,21 +31,27 @@ struct PoleMma {
   static constexpr int CPJ = (4 * L + 15) / 16;              // 16-byte chunks per operand (4L bytes)
   static constexpr int NCOL = 8 * L;                         // product columns
   static constexpr int NCT = NCOL / 16;                      // column tiles (L even)
-  static constexpr int MT = (NCT + 3) / 4;                   // column tiles per wave
+  static constexpr int MT = (NCT + 3) / 4;                   // column tiles per math wave
   static constexpr int P0 = NCOL - 1;                        // reversed digits: R[p] = d[P0 - p]
   static constexpr int RL4 = (NCOL + 16 * CPJ - 1 + 3) / 4 * 4;  // bytes per reversed copy
   static constexpr int DS = 4 * L + 4;                       // table bytes per constant entry
   static constexpr int CS = NCOL + 1;                        // LDS dwords per row of column sums (odd)
   static constexpr int LC = 64 * CS * 4;                     // column sums of 64 rows
-  // operands staged per batch: as many as fit in the column-sum area (6 for every compiled L), so that
-  // with S <= JB the staging is done before any accumulator is live
-  static constexpr int JB0 = (LC - 64 * 16) / (64 * 16 * CPJ + 4 * RL4);
-  static constexpr int JB = JB0 < 1 ? 1 : (JB0 > 16 ? 16 : JB0);
-  static constexpr int XROW = 16 * CPJ * JB + 16;            // LDS bytes per row: JB operands + pad
-  static constexpr int LX = 64 * XROW + 4 * JB * RL4 + JB * DS;  // staging: rows, 4 copies and the digits of
-                                                                  // JB operands
-  static constexpr int LDS = LX > LC ? LX : LC;
   static constexpr int SEGL = L / 2;                         // limbs per carry segment (4 segments)
+  // LDS of a task with J operands: column sums, the 64 rows' operands, 4 reversed copies of one constant
+  // row's digits, the digits and corrections of all J constant rows; then (own arrays) the constant rows'
+  // exponents, the row exponents and the segment carries
+  static constexpr int bytes(int J) {
+    return LC + 64 * (16 * CPJ * J + 16) + 4 * J * RL4 + J * J * DS + J * NCOL * 4 + 16 * 8 + 64 * 8 + 4 * 64 * 8;
+  }
+  static constexpr int jmax() {
+    int j = 1;
+    while (j < 16 && bytes(j + 1) <= 160 * 1024) ++j;
+    return j;
+  }
+  static constexpr int JB = jmax();                          // largest S the MFMA path takes (6 at L = 34)
+  static constexpr int XROW = 16 * CPJ * JB + 16;            // LDS bytes per staged row
+  static constexpr int LDS = bytes(JB) - 16 * 8 - 64 * 8 - 4 * 64 * 8;
 };
 
 // balanced base-256 digits of a two's complement constant (4L + 1 digits, then zero padding) and the
@@ -92,195 +97,211 @@ __global__ __launch_bounds__(256) void k_scorr(const int8_t* __restrict__ d8, in
 
 // X entry (xr, u, j, k) = Xm[xr S SB + u SB + j B + k]; output (orow, s0, n) = Om[orow S SB + s0 SB + n],
 // n = u B + k < SB = S B; constant row cr = ctab[task] S + s0, digits d8[(cr S + j) DS ..], corr[cr NCOL ..],
-// exponent Ce[cr].  xrow / orow null: the task index itself.  ONE: S <= JB, the operands staged once.
-// Two waves per SIMD (the LDS of two workgroups per CU): the compiler keeps the accumulators in VGPRs
-// within that budget rather than in AGPRs next to the epilogue's registers.
-template <int L, bool ONE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_pole_mma(const int32_t* __restrict__ Xm, const i64* __restrict__ Xe, i64 xps,
-                                                  const int64_t* __restrict__ xrow, const int32_t* __restrict__ ctab,
-                                                  const int8_t* __restrict__ d8, const int32_t* __restrict__ corr,
-                                                  const i64* __restrict__ Ce, const int64_t* __restrict__ orow,
-                                                  i64 T, int S, i64 B, int32_t* __restrict__ Om,
-                                                  i64* __restrict__ Oe, i64 ops) {
+// exponent Ce[cr].  xrow / orow null: the task index itself.  Requires S <= JB.
+// Work item = (task, block of 64 rows): the operands X are the same for the S constant rows s0, so they are
+// staged once, with the digits and corrections of all S rows; then waves 0-3 run the products of row s0
+// (reversed digit copies, MFMA, column sums into LDS) while waves 4-7 carry and round the column sums of
+// row s0 - 1.
+template <int L>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void k_pole_mma(
+    const int32_t* __restrict__ Xm, const i64* __restrict__ Xe, i64 xps, const int64_t* __restrict__ xrow,
+    const int32_t* __restrict__ ctab, const int8_t* __restrict__ d8, const int32_t* __restrict__ corr,
+    const i64* __restrict__ Ce, const int64_t* __restrict__ orow, i64 T, int S, i64 B, int32_t* __restrict__ Om,
+    i64* __restrict__ Oe, i64 ops) {
   using P = PoleMma<L>;
   __shared__ __align__(16) unsigned char lds[P::LDS];
+  __shared__ i64 Cex[16];                                             // [S s0] constant row exponents
+  __shared__ i64 Erow[64];
   __shared__ i64 cseg[4][64];
-  unsigned char* const X = lds;                     // [64][XROW]
-  unsigned char* const Dc = lds + 64 * P::XROW;     // [JB operands][4 copies][RL4]
-  int8_t* const Dr = (int8_t*)(Dc + 4 * P::JB * P::RL4);  // [JB operands][DS] digits as in the table
-  int32_t* const C = (int32_t*)lds;                 // [64][CS], after the products
+  int32_t* const C = (int32_t*)lds;                                   // [64][CS]
+  unsigned char* const X = lds + P::LC;                               // [64][XROW]
+  unsigned char* const Dc = X + 64 * P::XROW;                         // [JB operands][4 copies][RL4]
+  int8_t* const Dr = (int8_t*)(Dc + 4 * P::JB * P::RL4);              // [S s0][S j][DS]
+  int32_t* const Cr = (int32_t*)(Dr + P::JB * P::JB * P::DS);         // [S s0][NCOL] (DS: whole dwords)
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const bool math = w < 4;
   const int r = lane & 15, g = lane >> 4;
   const i64 SB = (i64)S * B;
   const i64 NRB = (SB + 63) / 64;
-  const i64 items = T * S * NRB;
+  const i64 items = T * NRB;
+  const int nch = S * P::CPJ;
+  const int nks = (nch + 3) / 4;
   for (i64 it = blockIdx.x; it < items; it += gridDim.x) {
-    const i64 grp = it / NRB, rb = it - grp * NRB;
-    const i64 task = grp / S;
-    const int s0 = (int)(grp - task * S);
-    const i64 cr = (i64)ctab[task] * S + s0;
+    const i64 task = it / NRB, rb = it - task * NRB;
+    const i64 cr0 = (i64)ctab[task] * S;
     const i64 xb = (xrow ? xrow[task] : task) * S * SB;
-    // this thread's row (each wave covers the 64 rows and stages every fourth operand)
-    const i64 n = rb * 64 + lane;
+    const i64 ob = (orow ? orow[task] : task) * S * SB;
+    const i64 n = rb * 64 + lane;                  // this thread's row
     const bool valid = n < SB;
     const i64 u = valid ? n / B : 0, k = valid ? n - (n / B) * B : 0;
     const i64 xrowbase = xb + u * SB + k;
-    i64 E = B3D_EZ;   // zero entries carry EZ
+    i64 E = B3D_EZ;                                // zero entries carry EZ
 #pragma unroll
-    for (int j = 0; j < 16; ++j)   // S <= 16: every exponent load in flight at once
+    for (int j = 0; j < 16; ++j)                   // S <= 16: every exponent load in flight at once
       if (j < S) {
         const i64 ej = valid ? Xe[xrowbase + j * B] : B3D_EZ;
         E = ej > E ? ej : E;
       }
-    // column corrections: the accumulators start from them (the same for every row)
-    int32_t cinit[P::MT];
-#pragma unroll
-    for (int mt = 0; mt < P::MT; ++mt) {
-      const int ct = w + 4 * mt < P::NCT ? w + 4 * mt : P::NCT - 1;
-      cinit[mt] = corr[cr * P::NCOL + 16 * ct + r];
-    }
-    const i64 ea = Ce[cr];
-    __syncthreads();   // the previous item's epilogue is done with LDS
-    v4i acc[4][P::MT];
-    for (int jb = 0; ONE ? jb < 1 : jb * P::JB < S; ++jb) {
-      const int nb = S - P::JB * jb < P::JB ? S - P::JB * jb : P::JB;
-      if (jb > 0) __syncthreads();   // the previous batch's fragments are read
-      // A: row `lane`, operands jl = w, w + 4, .. of the batch: bytes u ^ 0x80 except the signed top byte,
-      // zero padding
-      for (int jl = w; jl < nb; jl += 4) {
-        const int j = P::JB * jb + jl;
-        u32* row = (u32*)(X + lane * P::XROW + jl * 16 * P::CPJ);
-        {
-          u32 x[L];
+    __syncthreads();                               // the previous item is done with LDS
+    if (w == 0) Erow[lane] = E;
+    // operands: row `lane`, operand jl = w, w + 8, ..: bytes u ^ 0x80 except the signed top byte, zero padding
+    for (int jl = w; jl < S; jl += 8) {
+      u32* row = (u32*)(X + lane * P::XROW + jl * 16 * P::CPJ);
+      u32 x[L];
+      mp::zero(x);
+      if (valid && E != B3D_EZ) {
+        const i64 idx = xrowbase + (i64)jl * B;
+        mp::load(x, Xm, xps, idx);
+        const i64 sh = E - Xe[idx];
+        if (x[L - 1] != 0u && sh < 32 * L) {
+          const u32 sm = mp::to_mag(x);
+          mp::shr(x, (int)sh);
+          mp::cneg(x, sm);
+        } else {
           mp::zero(x);
-          if (valid && E != B3D_EZ) {
-            const i64 idx = xrowbase + (i64)j * B;
-            mp::load(x, Xm, xps, idx);
-            const i64 sh = E - Xe[idx];
-            if (x[L - 1] != 0u && sh < 32 * L) {
-              const u32 sm = mp::to_mag(x);
-              mp::shr(x, (int)sh);
-              mp::cneg(x, sm);
-            } else {
-              mp::zero(x);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 4 * P::CPJ; ++q)
+        row[q] = q < L - 1 ? (x[q] ^ 0x80808080u) : (q == L - 1 ? (x[q] ^ 0x00808080u) : 0u);
+    }
+    // the task's S x S digit rows and S correction rows (contiguous in the tables), S row exponents
+    {
+      const u32* src = (const u32*)(d8 + cr0 * S * P::DS);
+      for (int q = tid; q < S * S * (P::DS / 4); q += 512) ((u32*)Dr)[q] = src[q];
+      const int32_t* cs = corr + cr0 * P::NCOL;
+      for (int q = tid; q < S * P::NCOL; q += 512) Cr[q] = cs[q];
+      if (tid < S) Cex[tid] = Ce[cr0 + tid];
+    }
+    __syncthreads();
+    for (int step = 0; step <= S; ++step) {
+      v4i acc[4][P::MT];
+      // phase A: math waves build the reversed copies of row `step`; epilogue waves carry row step - 1
+      if (math) {
+        if (step < S)
+#pragma unroll 4
+          for (int q = tid; q < S * 4 * (P::RL4 / 4); q += 256) {
+            const int jl = q / (4 * (P::RL4 / 4));
+            const int rem = q - jl * 4 * (P::RL4 / 4);
+            const int sft = rem / (P::RL4 / 4), wd = rem - sft * (P::RL4 / 4);
+            const int8_t* d = Dr + (step * S + jl) * P::DS;
+            u32 v = 0u;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+              const int di = P::P0 - 4 * wd - sft - b;
+              const u32 byte = (di >= 0 && di <= 4 * L) ? (u32)(uint8_t)d[di] : 0u;
+              v |= byte << (8 * b);
             }
+            ((u32*)(Dc + (jl * 4 + sft) * P::RL4))[wd] = v;
           }
+      } else if (step >= 1 && valid) {
+        // row `lane`, segment w - 4 = columns [2L (w-4), 2L (w-4) + 2L) -> L/2 limbs in place + a carry
+        const int sg = w - 4;
+        int32_t* crow = C + lane * P::CS + 2 * L * sg;
+        // all of the segment's column sums are read first (one LDS wait), then carried
+        i64 part[P::SEGL];
 #pragma unroll
-          for (int q = 0; q < 4 * P::CPJ; ++q)
-            row[q] = q < L - 1 ? (x[q] ^ 0x80808080u) : (q == L - 1 ? (x[q] ^ 0x00808080u) : 0u);
+        for (int q = 0; q < P::SEGL; ++q) {
+          i64 v = 0;
+#pragma unroll
+          for (int b = 0; b < 4; ++b) v += (i64)crow[4 * q + b] * ((i64)1 << (8 * b));
+          part[q] = v;
         }
-      }
-      // the batch's digit rows (contiguous in the table) into LDS, one dword per thread
-      {
-        const u32* src = (const u32*)(d8 + (cr * S + P::JB * jb) * P::DS);
-        for (int q = tid; q < nb * (P::DS / 4); q += 256) ((u32*)Dr)[q] = src[q];
+        i64 carry = 0;
+#pragma unroll
+        for (int q = 0; q < P::SEGL; ++q) {
+          const i64 v = part[q] + carry;
+          crow[q] = (int32_t)(u32)v;
+          carry = v >> 32;
+        }
+        cseg[sg][lane] = carry;
       }
       __syncthreads();
-      // B source: operand jl's digits reversed, four copies shifted by 0..3 bytes:
-      // copy s, byte p' = d[P0 - p' - s] (zero outside 0 .. 4L)
-      for (int q = tid; q < nb * 4 * (P::RL4 / 4); q += 256) {
-        const int jl = q / (4 * (P::RL4 / 4));
-        const int rem = q - jl * 4 * (P::RL4 / 4);
-        const int s = rem / (P::RL4 / 4), wd = rem - s * (P::RL4 / 4);
-        const int8_t* d = Dr + jl * P::DS;
-        u32 v = 0u;
+      // phase B: math waves multiply row `step`; epilogue wave 4 rounds row step - 1
+      if (math) {
+        if (step < S) {
+          const int32_t* cc = Cr + step * P::NCOL;
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-          const int di = P::P0 - 4 * wd - s - b;
-          const u32 byte = (di >= 0 && di <= 4 * L) ? (u32)(uint8_t)d[di] : 0u;
-          v |= byte << (8 * b);
+          for (int mt = 0; mt < P::MT; ++mt) {
+            const int ct = w + 4 * mt < P::NCT ? w + 4 * mt : P::NCT - 1;
+            const int c0 = cc[16 * ct + r];          // the accumulators start from the corrections
+#pragma unroll
+            for (int rt = 0; rt < 4; ++rt) acc[rt][mt] = v4i{c0, c0, c0, c0};
+          }
+          for (int ks = 0; ks < nks; ++ks) {
+            const int q = 4 * ks + g;   // this lane's chunk: operand q / CPJ, bytes 16 (q % CPJ) ..
+            v4i A[4];
+#pragma unroll
+            for (int rt = 0; rt < 4; ++rt) A[rt] = *(const v4i*)(X + (16 * rt + r) * P::XROW + 16 * q);
+            const int jl = q / P::CPJ;
+            const int i0 = 16 * (q - jl * P::CPJ);
+            // padding chunks past the operands: B fragment 0 (loads from a valid operand, masked)
+            const int lmask = jl < S ? -1 : 0;
+            const int jc = jl < S ? jl : S - 1;
+            // every fragment of the K-step is read before the products, one LDS wait per K-step
+            v4i Bf[P::MT];
+#pragma unroll
+            for (int mt = 0; mt < P::MT; ++mt) {
+              // column tile (the waves with one tile less repeat their last one; never written back)
+              const int ct = w + 4 * mt < P::NCT ? w + 4 * mt : P::NCT - 1;
+              const int o = P::P0 - (16 * ct + r) + i0;   // R[o + b] = d[c - i0 - b]
+              const int sft = o & 3;
+              const u32* src = (const u32*)(Dc + (jc * 4 + sft) * P::RL4 + (o - sft));
+              Bf[mt] = v4i{(int)src[0] & lmask, (int)src[1] & lmask, (int)src[2] & lmask, (int)src[3] & lmask};
+            }
+#pragma unroll
+            for (int mt = 0; mt < P::MT; ++mt)
+#pragma unroll
+              for (int rt = 0; rt < 4; ++rt)
+                acc[rt][mt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[rt], Bf[mt], acc[rt][mt], 0, 0, 0);
+          }
         }
-        ((u32*)(Dc + (jl * 4 + s) * P::RL4))[wd] = v;
+      } else if (w == 4 && step >= 1 && valid) {
+        const int s0 = step - 1;
+        u32 W[2 * L + 2];
+#pragma unroll
+        for (int sg = 0; sg < 4; ++sg)
+#pragma unroll
+          for (int q = 0; q < P::SEGL; ++q) W[sg * P::SEGL + q] = (u32)C[lane * P::CS + 2 * L * sg + q];
+        const i64 c3 = cseg[3][lane];
+        W[2 * L] = (u32)c3;
+        W[2 * L + 1] = (u32)((u64)c3 >> 32);
+#pragma unroll
+        for (int sg = 0; sg < 3; ++sg) {
+          const i64 cs = cseg[sg][lane];
+          const u32 ext = cs < 0 ? 0xFFFFFFFFu : 0u;
+          u32 cy = 0u;
+#pragma unroll
+          for (int q = (sg + 1) * P::SEGL; q < 2 * L + 2; ++q)
+            W[q] = mp::addc(W[q], q == (sg + 1) * P::SEGL ? (u32)cs : (q == (sg + 1) * P::SEGL + 1 ? (u32)((u64)cs >> 32) : ext), cy);
+        }
+        const i64 ea = Cex[s0], Er = Erow[lane];
+        u32 out[L];
+        i64 e;
+        if (Er == B3D_EZ || ea == B3D_EZ) {
+          mp::zero(out);
+          e = B3D_EZ;
+        } else {
+          e = mp::norm<L, 2 * L + 2>(W, ea + Er, out);
+        }
+        const i64 o = ob + s0 * SB + n;
+        mp::store(out, Om, ops, o);
+        Oe[o] = e;
       }
       __syncthreads();
-      if (jb == 0) {   // (ONE: no accumulator is live while the operands are staged)
+      // phase C: column sums of row `step` into LDS (the epilogue has read row step - 1's)
+      if (math && step < S) {
 #pragma unroll
         for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
-          for (int mt = 0; mt < P::MT; ++mt) acc[rt][mt] = v4i{cinit[mt], cinit[mt], cinit[mt], cinit[mt]};
+          for (int mt = 0; mt < P::MT; ++mt) {
+            const int ct = w + 4 * mt;
+            if (ct < P::NCT)
+#pragma unroll
+              for (int i = 0; i < 4; ++i) C[(16 * rt + 4 * g + i) * P::CS + 16 * ct + r] = acc[rt][mt][i];
+          }
       }
-      const int nch = nb * P::CPJ;
-      const int nks = (nch + 3) / 4;
-      for (int ks = 0; ks < nks; ++ks) {
-        const int q = 4 * ks + g;   // this lane's chunk: operand q / CPJ, bytes 16 (q % CPJ) ..
-        v4i A[4];
-#pragma unroll
-        for (int rt = 0; rt < 4; ++rt) A[rt] = *(const v4i*)(X + (16 * rt + r) * P::XROW + 16 * q);
-        const int jl = q / P::CPJ;
-        const int i0 = 16 * (q - jl * P::CPJ);
-        const bool live = jl < nb;                  // padding chunks past the batch: B fragment 0
-        const int jc = live ? jl : nb - 1;
-#pragma unroll
-        for (int mt = 0; mt < P::MT; ++mt) {
-          // column tile (the waves with one tile less repeat their last one; never written back)
-          const int ct = w + 4 * mt < P::NCT ? w + 4 * mt : P::NCT - 1;
-          const int o = P::P0 - (16 * ct + r) + i0;   // R[o + b] = d[c - i0 - b]
-          const int sft = o & 3;
-          const u32* src = (const u32*)(Dc + (jc * 4 + sft) * P::RL4 + (o - sft));
-          const v4i Bf = v4i{live ? (int)src[0] : 0, live ? (int)src[1] : 0, live ? (int)src[2] : 0,
-                             live ? (int)src[3] : 0};
-#pragma unroll
-          for (int rt = 0; rt < 4; ++rt) acc[rt][mt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[rt], Bf, acc[rt][mt], 0, 0, 0);
-        }
-      }
-    }
-    __syncthreads();   // products done: the column sums overwrite the staging area
-#pragma unroll
-    for (int rt = 0; rt < 4; ++rt)
-#pragma unroll
-      for (int mt = 0; mt < P::MT; ++mt) {
-        const int ct = w + 4 * mt;
-        if (ct < P::NCT)
-#pragma unroll
-          for (int i = 0; i < 4; ++i) C[(16 * rt + 4 * g + i) * P::CS + 16 * ct + r] = acc[rt][mt][i];
-        __builtin_amdgcn_sched_barrier(0);   // tile by tile: the accumulators are not all read out at once
-      }
-    __syncthreads();
-    // carry pass: row `lane`, segment w = columns [2L w, 2L w + 2L) -> L/2 limbs written in place + a carry
-    if (valid) {
-      int32_t* cr_row = C + lane * P::CS + 2 * L * w;
-      i64 carry = 0;
-#pragma unroll 2
-      for (int q = 0; q < P::SEGL; ++q) {
-        i64 v = carry;
-#pragma unroll
-        for (int b = 0; b < 4; ++b) v += (i64)cr_row[4 * q + b] * ((i64)1 << (8 * b));
-        cr_row[q] = (int32_t)(u32)v;
-        carry = v >> 32;
-      }
-      cseg[w][lane] = carry;
-    }
-    __syncthreads();
-    // assembly and rounding: one thread per row
-    if (w == 0 && valid) {
-      u32 W[2 * L + 2];
-#pragma unroll
-      for (int s = 0; s < 4; ++s)
-#pragma unroll
-        for (int q = 0; q < P::SEGL; ++q) W[s * P::SEGL + q] = (u32)C[lane * P::CS + 2 * L * s + q];
-      const i64 c3 = cseg[3][lane];
-      W[2 * L] = (u32)c3;
-      W[2 * L + 1] = (u32)((u64)c3 >> 32);
-#pragma unroll
-      for (int s = 0; s < 3; ++s) {
-        const i64 cs = cseg[s][lane];
-        const u32 ext = cs < 0 ? 0xFFFFFFFFu : 0u;
-        u32 cy = 0u;
-#pragma unroll
-        for (int q = (s + 1) * P::SEGL; q < 2 * L + 2; ++q)
-          W[q] = mp::addc(W[q], q == (s + 1) * P::SEGL ? (u32)cs : (q == (s + 1) * P::SEGL + 1 ? (u32)((u64)cs >> 32) : ext), cy);
-      }
-      u32 out[L];
-      i64 e;
-      if (E == B3D_EZ || ea == B3D_EZ) {
-        mp::zero(out);
-        e = B3D_EZ;
-      } else {
-        e = mp::norm<L, 2 * L + 2>(W, ea + E, out);
-      }
-      const i64 o = (orow ? orow[task] : task) * S * SB + s0 * SB + n;
-      mp::store(out, Om, ops, o);
-      Oe[o] = e;
+      __syncthreads();
     }
   }
 }
@@ -288,11 +309,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 int b3d::pole_mma(int L, const int32_t* Xm, const int64_t* Xe, int64_t xps, const int64_t* xrow, const int32_t* ctab,
                   const int8_t* d8, const int32_t* corr, const int64_t* Ce, const int64_t* orow, int64_t T, int S,
                   int64_t B, int32_t* Om, int64_t* Oe, int64_t ops, hipStream_t s) {
-  const int64_t items = T * S * ((S * B + 63) / 64);
+  const int64_t items = T * ((S * B + 63) / 64);
   const unsigned grid = (unsigned)(items < (1 << 22) ? items : (1 << 22));
-  B3D_DISPATCH_L(L, if (S <= PoleMma<LL>::JB) k_pole_mma<LL, true><<<dim3(grid), dim3(256), 0, s>>>(Xm, Xe, xps, xrow, ctab, d8, corr, Ce, orow, T, S, B, Om, Oe, ops);
-                    else k_pole_mma<LL, false><<<dim3(grid), dim3(256), 0, s>>>(Xm, Xe, xps, xrow, ctab, d8, corr, Ce, orow, T, S, B, Om, Oe, ops));
+  B3D_DISPATCH_L(L, k_pole_mma<LL><<<dim3(grid), dim3(512), 0, s>>>(Xm, Xe, xps, xrow, ctab, d8, corr, Ce, orow, T, S, B, Om, Oe, ops));
   return b3d::check_launch("b3d_pole_level (MFMA)");
+}
+
+int b3d::pole_mma_max_s(int L) {
+  B3D_DISPATCH_L(L, return PoleMma<LL>::JB);
+  return 0;
 }
 
 extern "C" {

@@ -34,7 +34,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L <= 34 ? 5
     const i64* __restrict__ he, i64 hps, const int64_t* __restrict__ ghrow, const int32_t* __restrict__ rg,
     const int32_t* __restrict__ rd, i64 nH, const int64_t* __restrict__ col, const int32_t* __restrict__ cold,
     const int32_t* __restrict__ ncol, int Kc, int32_t* __restrict__ om, i64* __restrict__ oe, i64 ops, i64 W) {
-  for (i64 u = blockIdx.x * (i64)blockDim.x + threadIdx.x; u < nH * W; u += (i64)gridDim.x * blockDim.x) {
+  const unsigned lb = b3d_xcd_block(blockIdx.x, gridDim.x);   // a group's rows stay on one XCD's L2
+  for (i64 u = lb * (i64)blockDim.x + threadIdx.x; u < nH * W; u += (i64)gridDim.x * blockDim.x) {
     const i64 n = u % W;
     const int h = __builtin_amdgcn_readfirstlane((int)(u / W));
     if (n >= N) continue;
@@ -57,7 +58,8 @@ __global__ __launch_bounds__(256) void k_cauchy(const int32_t* __restrict__ rm, 
                                                 const int64_t* __restrict__ col, const int32_t* __restrict__ cold,
                                                 const int32_t* __restrict__ ncol, int Kc, int32_t* __restrict__ om,
                                                 i64* __restrict__ oe, i64 ops) {
-  for (i64 t = blockIdx.x * (i64)blockDim.x + threadIdx.x; t < nH * N; t += (i64)gridDim.x * blockDim.x) {
+  const unsigned lb = b3d_xcd_block(blockIdx.x, gridDim.x);   // a group's rows stay on one XCD's L2
+  for (i64 t = lb * (i64)blockDim.x + threadIdx.x; t < nH * N; t += (i64)gridDim.x * blockDim.x) {
     const i64 n = t % N, h = t / N;
     const int g = rg[h];
     u32 r[L];
@@ -75,7 +77,8 @@ __global__ __launch_bounds__(256) void k_pole_r(const int32_t* __restrict__ Hm, 
                                                 const uint32_t* __restrict__ Rsg, const i64* __restrict__ Re,
                                                 i64 rps, int32_t* __restrict__ Ym, i64* __restrict__ Ye, i64 yps) {
   const i64 SB = (i64)S * B;
-  for (i64 t = blockIdx.x * (i64)blockDim.x + threadIdx.x; t < T * S * SB; t += (i64)gridDim.x * blockDim.x) {
+  const unsigned lb = b3d_xcd_block(blockIdx.x, gridDim.x);   // a task's outputs stay on one XCD's L2
+  for (i64 t = lb * (i64)blockDim.x + threadIdx.x; t < T * S * SB; t += (i64)gridDim.x * blockDim.x) {
     const i64 k = t % B;
     const i64 a = (t / B) % S;
     const i64 b0 = (t / SB) % S;
@@ -96,7 +99,8 @@ __global__ __launch_bounds__(256) void k_pole_l(const int32_t* __restrict__ Ym, 
                                                 const uint32_t* __restrict__ Qsg, const i64* __restrict__ Qe, i64 qps,
                                                 int32_t* __restrict__ rm, i64* __restrict__ re, i64 rps) {
   const i64 SB = (i64)S * B;
-  for (i64 t = blockIdx.x * (i64)blockDim.x + threadIdx.x; t < T * S * SB; t += (i64)gridDim.x * blockDim.x) {
+  const unsigned lb = b3d_xcd_block(blockIdx.x, gridDim.x);   // a task's outputs stay on one XCD's L2
+  for (i64 t = lb * (i64)blockDim.x + threadIdx.x; t < T * S * SB; t += (i64)gridDim.x * blockDim.x) {
     const i64 k = t % B;
     const i64 b0 = (t / B) % S;
     const i64 a0 = (t / SB) % S;
@@ -143,7 +147,7 @@ int b3d_pole_level(int L, int S, int64_t B, int64_t T, const int32_t* Hm, const 
   }
   if (b3d::check_planes("b3d_pole_level", {hps, rtps, qps, yps, rps})) return 2;
   hipStream_t s = (hipStream_t)stream;
-  if (Rd8 && Rcorr && Qd8 && Qcorr && (int64_t)S * B >= 32) {
+  if (Rd8 && Rcorr && Qd8 && Qcorr && (int64_t)S * B >= 32 && S <= b3d::pole_mma_max_s(L)) {
     // at least 32 rows share each constant row: int8 MFMA products
     if (b3d::pole_mma(L, Hm, He, hps, thrw, tR, Rd8, Rcorr, Re, nullptr, T, S, B, Ym, Ye, yps, s)) return 1;
     return b3d::pole_mma(L, Ym, Ye, yps, nullptr, tQL, Qd8, Qcorr, Qe, tslot, T, S, B, rm, re, rps, s);

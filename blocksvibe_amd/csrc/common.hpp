@@ -30,7 +30,18 @@ inline int check_planes(const char* what, std::initializer_list<int64_t> strides
 int pole_mma(int L, const int32_t* Xm, const int64_t* Xe, int64_t xps, const int64_t* xrow, const int32_t* ctab,
              const int8_t* d8, const int32_t* corr, const int64_t* Ce, const int64_t* orow, int64_t T, int S, int64_t B,
              int32_t* Om, int64_t* Oe, int64_t ops, hipStream_t s);
+// largest structure count S the MFMA pole step takes at L limbs (its operands and digit rows fit in LDS)
+int pole_mma_max_s(int L);
 }  // namespace b3d
+
+// Workgroups are dealt to the 8 XCDs (each with its own L2) round robin by dispatch index.  This maps
+// dispatch index b of a G-workgroup grid to a logical block so that each XCD runs one contiguous range of
+// logical blocks: neighbouring blocks, which read the same data (the rows of one Cauchy group), share an L2.
+// A bijection of [0, G).
+__device__ __forceinline__ unsigned b3d_xcd_block(unsigned b, unsigned G) {
+  const unsigned xcd = b & 7u, idx = b >> 3, per = G >> 3, rem = G & 7u;
+  return xcd < rem ? xcd * (per + 1u) + idx : rem * (per + 1u) + (xcd - rem) * per + idx;
+}
 
 // Limb counts with compiled kernels.  L = 2 ceil(precision / 64) + 2 (precision rounded up to whole
 // 64-bit limbs, plus >= 60 guard bits): every precision that is a multiple of 128 bits up to 1536 bits,

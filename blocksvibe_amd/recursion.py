@@ -9,6 +9,7 @@ over all right-hand sides (four-point structures x lambda orders) at once: the b
 dimension of every array.  Constants (M matrices, Q values, their pre-aligned forms) are built once per
 operator and reused by every run().
 """
+import os
 import time
 from fractions import Fraction
 
@@ -106,8 +107,8 @@ class Recursion:
         return r
 
     # pole-step products on int8 MFMA where at least 32 rows share a constant row (bit-identical to the VALU
-    # path; off by default while its kernel is slower than the VALU one at the benchmark's shapes)
-    mma = False
+    # path); BLOCKSVIBE_POLE_MMA=1 / 0 selects the default
+    mma = os.environ.get("BLOCKSVIBE_POLE_MMA", "0") == "1"
 
     def _constants(self):
         plan, Sm, L = self.plan, self.plan.Smax, self.L

@@ -87,7 +87,8 @@ int b3d_cauchy_level(int L, const int32_t* rm, const int64_t* re, int64_t rps, i
  * the row-aligned mantissas: 28-bit magnitude limb planes R28/Q28 (b3d_limbs28(L) planes of (ntab, S, S),
  * plane strides rtps/qps) and sign masks Rsg/Qsg (ntab, S, S), with row exponents Re/Qe (ntab, S); and in
  * the form b3d_pole_tables makes (digits Rd8/Qd8, corrections Rcorr/Qcorr; may be null).  With the digit
- * tables and S*B >= 32 the products run on int8 MFMA, otherwise on the VALU; both give the same bits.
+ * tables, S*B >= 32 and S small enough for the MFMA path's LDS staging (6 at L = 34) the products run on
+ * int8 MFMA, otherwise on the VALU; both give the same bits.
  * Y: scratch (L, T, S, S, B), plane stride yps. */
 int b3d_pole_level(int L, int S, int64_t B, int64_t T, const int32_t* Hm, const int64_t* He, int64_t hps,
                    const int64_t* thrw, const int32_t* tR, const int32_t* tQL, const int64_t* tslot,

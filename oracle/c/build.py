@@ -10,7 +10,7 @@ LIB = os.path.join(HERE, "liboracle_c.so")
 
 
 def build():
-    tmp = LIB + ".tmp"
+    tmp = f"{LIB}.{os.getpid()}.tmp"     # concurrent builds (parallel test workers) never share a file
     subprocess.run(["gcc", "-O3", "-fPIC", "-shared", "-std=c11", "-D_POSIX_C_SOURCE=200809L", "-Wall",
                     "-o", tmp, os.path.join(HERE, "mpf_c.c"), "-lpthread"], check=True)
     os.replace(tmp, LIB)

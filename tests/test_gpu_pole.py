@@ -1,9 +1,10 @@
 """The pole step's two product paths through the C-ABI (b3d_pole_level): the int8 MFMA path (digit tables,
 S*B >= 32 rows per constant row) against the VALU path (28-bit limbs, no digit tables) on the same random
 inputs, bit for bit, for both products (Y = R H^T and res = QL Y^T).  Covers several row blocks per
-constant row, more operands than one LDS batch holds (S = 8), zero entries and exponent spreads wide
-enough that aligned entries vanish.  The VALU path itself is checked against the oracle bit for bit in
-tests/test_gpu_recursion.py; the digit arithmetic against exact integers in tests/test_pole_digits.py."""
+task, S up to the most the MFMA path stages in LDS (6 at L = 34, 3 at L = 50, 16 at L = 4), S = 8 at
+L = 34 (taken by the VALU path), zero entries and exponent spreads wide enough that aligned entries
+vanish.  The VALU path itself is checked against the oracle bit for bit in tests/test_gpu_recursion.py;
+the digit arithmetic against exact integers in tests/test_pole_digits.py."""
 import numpy as np
 import pytest
 import torch
@@ -21,7 +22,8 @@ def _rand(shape, L, seed, zero_frac=0.1, spread=40):
     return mpf.MPF(m, e)
 
 
-@pytest.mark.parametrize("L,S,B", [(34, 5, 11), (34, 8, 20), (10, 6, 7), (50, 4, 9), (4, 5, 13)])
+@pytest.mark.parametrize("L,S,B", [(34, 5, 11), (34, 6, 20), (34, 8, 5), (10, 6, 7), (50, 3, 12), (4, 5, 13),
+                                   (4, 16, 3)])
 def test_pole_mma_matches_valu(L, S, B):
     from blocksvibe_amd import _lib, mpf
     from blocksvibe_amd.recursion import _pole_tables, _split28

@@ -23,7 +23,7 @@ P, I64 = ctypes.c_void_p, ctypes.c_int64
 def lib():
     if not os.path.exists(CLANG):
         pytest.skip("clang++ from ROCm is needed for the host build of mp.hpp")
-    tmp = SO + ".tmp"
+    tmp = f"{SO}.{os.getpid()}.tmp"
     subprocess.run([CLANG, "-O1", "-std=c++17", "-fPIC", "-shared", "-o", tmp,
                     os.path.join(HERE, "host", "mp_host.cpp")], check=True)
     os.replace(tmp, SO)
