@@ -38,11 +38,16 @@ struct PoleMma {
   static constexpr int CS = NCOL + 1;                        // LDS dwords per row of column sums (odd)
   static constexpr int LC = 64 * CS * 4;                     // column sums of 64 rows
   static constexpr int SEGL = L / 2;                         // limbs per carry segment (4 segments)
+  // K order of the products: the 16-byte chunk ci of operand j sits in K-step ks, lane group g (a K-step is
+  // 4 chunks).  Whole groups of 4 operands share a K-step per ci (ks = (j / 4) CPJ + ci, g = j % 4), so the
+  // K-step's Toeplitz tiles are zero outside column tiles ci .. ci + CPJ; the remaining operands follow with
+  // their chunks packed in order (slot = (j - 4 FG) CPJ + ci, ks = FG CPJ + slot / 4, g = slot % 4).
+  static constexpr int nks(int J) { return (J / 4) * CPJ + ((J % 4) * CPJ + 3) / 4; }
   // LDS of a task with J operands: column sums, the 64 rows' operands, 4 reversed copies of one constant
   // row's digits, the digits and corrections of all J constant rows; then (own arrays) the constant rows'
-  // exponents, the row exponents and the segment carries
+  // exponents and the row exponents
   static constexpr int bytes(int J) {
-    return LC + 64 * (16 * CPJ * J + 16) + 4 * J * RL4 + J * J * DS + J * NCOL * 4 + 16 * 8 + 64 * 8 + 4 * 64 * 8;
+    return LC + 64 * (64 * nks(J) + 16) + 4 * J * RL4 + J * J * DS + J * NCOL * 4 + 16 * 8 + 64 * 8;
   }
   static constexpr int jmax() {
     int j = 1;
@@ -50,8 +55,8 @@ struct PoleMma {
     return j;
   }
   static constexpr int JB = jmax();                          // largest S the MFMA path takes (6 at L = 34)
-  static constexpr int XROW = 16 * CPJ * JB + 16;            // LDS bytes per staged row
-  static constexpr int LDS = bytes(JB) - 16 * 8 - 64 * 8 - 4 * 64 * 8;
+  static constexpr int XROW = 64 * nks(JB) + 16;             // LDS bytes per staged row
+  static constexpr int LDS = bytes(JB) - 16 * 8 - 64 * 8;
 };
 
 // balanced base-256 digits of a two's complement constant (4L + 1 digits, then zero padding) and the
@@ -112,7 +117,6 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void k
   __shared__ __align__(16) unsigned char lds[P::LDS];
   __shared__ i64 Cex[16];                                             // [S s0] constant row exponents
   __shared__ i64 Erow[64];
-  __shared__ i64 cseg[4][64];
   int32_t* const C = (int32_t*)lds;                                   // [64][CS]
   unsigned char* const X = lds + P::LC;                               // [64][XROW]
   unsigned char* const Dc = X + 64 * P::XROW;                         // [JB operands][4 copies][RL4]
@@ -124,8 +128,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void k
   const i64 SB = (i64)S * B;
   const i64 NRB = (SB + 63) / 64;
   const i64 items = T * NRB;
-  const int nch = S * P::CPJ;
-  const int nks = (nch + 3) / 4;
+  const int nks = P::nks(S);
   for (i64 it = blockIdx.x; it < items; it += gridDim.x) {
     const i64 task = it / NRB, rb = it - task * NRB;
     const i64 cr0 = (i64)ctab[task] * S;
@@ -145,8 +148,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void k
     __syncthreads();                               // the previous item is done with LDS
     if (w == 0) Erow[lane] = E;
     // operands: row `lane`, operand jl = w, w + 8, ..: bytes u ^ 0x80 except the signed top byte, zero padding
+    const int FG = S / 4;
     for (int jl = w; jl < S; jl += 8) {
-      u32* row = (u32*)(X + lane * P::XROW + jl * 16 * P::CPJ);
+      unsigned char* row = X + lane * P::XROW;
       u32 x[L];
       mp::zero(x);
       if (valid && E != B3D_EZ) {
@@ -162,8 +166,23 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void k
         }
       }
 #pragma unroll
-      for (int q = 0; q < 4 * P::CPJ; ++q)
-        row[q] = q < L - 1 ? (x[q] ^ 0x80808080u) : (q == L - 1 ? (x[q] ^ 0x00808080u) : 0u);
+      for (int ci = 0; ci < P::CPJ; ++ci) {
+        int ks, gg;
+        if (jl < 4 * FG) {
+          ks = (jl >> 2) * P::CPJ + ci;
+          gg = jl & 3;
+        } else {
+          const int slot = (jl - 4 * FG) * P::CPJ + ci;
+          ks = FG * P::CPJ + (slot >> 2);
+          gg = slot & 3;
+        }
+        u32* dst = (u32*)(row + (ks * 4 + gg) * 16);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const int qd = 4 * ci + t;
+          dst[t] = qd < L - 1 ? (x[qd] ^ 0x80808080u) : (qd == L - 1 ? (x[qd] ^ 0x00808080u) : 0u);
+        }
+      }
     }
     // the task's S x S digit rows and S correction rows (contiguous in the tables), S row exponents
     {
@@ -173,51 +192,154 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void k
       for (int q = tid; q < S * P::NCOL; q += 512) Cr[q] = cs[q];
       if (tid < S) Cex[tid] = Ce[cr0 + tid];
     }
-    __syncthreads();
-    for (int step = 0; step <= S; ++step) {
-      v4i acc[4][P::MT];
-      // phase A: math waves build the reversed copies of row `step`; epilogue waves carry row step - 1
-      if (math) {
-        if (step < S)
-#pragma unroll 4
-          for (int q = tid; q < S * 4 * (P::RL4 / 4); q += 256) {
-            const int jl = q / (4 * (P::RL4 / 4));
-            const int rem = q - jl * 4 * (P::RL4 / 4);
-            const int sft = rem / (P::RL4 / 4), wd = rem - sft * (P::RL4 / 4);
-            const int8_t* d = Dr + (step * S + jl) * P::DS;
-            u32 v = 0u;
+    // reversed digit copies of constant row s0 (math waves): copy sft, byte p' = d[P0 - p' - sft], zero outside
+    // 0 .. 4L.  P0 = 8L - 1 = 3 mod 4, so the reversed string's dword m is the byte swap of the digits' dword
+    // 2L - 1 - m, and copy sft's dword wd is the byte funnel of reversed dwords wd + 1 : wd by sft.
+    auto build_copies = [&](int s0) {
+      for (int q = tid; q < S * (P::RL4 / 4); q += 256) {
+        const int jl = q / (P::RL4 / 4), wd = q - jl * (P::RL4 / 4);
+        const u32* dw = (const u32*)(Dr + (s0 * S + jl) * P::DS);    // DS / 4 = L + 1 dwords, digit 4L in the last
+        const int i0 = 2 * L - 1 - wd, i1 = 2 * L - 2 - wd;           // digit dwords of reversed dwords wd, wd + 1
+        const u32 r0 = (i0 >= 0 && i0 <= L) ? __builtin_bswap32(dw[i0]) : 0u;
+        const u32 r1 = (i1 >= 0 && i1 <= L) ? __builtin_bswap32(dw[i1]) : 0u;
+        u32* dst = (u32*)(Dc + jl * 4 * P::RL4) + wd;
+        dst[0] = r0;
 #pragma unroll
-            for (int b = 0; b < 4; ++b) {
-              const int di = P::P0 - 4 * wd - sft - b;
-              const u32 byte = (di >= 0 && di <= 4 * L) ? (u32)(uint8_t)d[di] : 0u;
-              v |= byte << (8 * b);
-            }
-            ((u32*)(Dc + (jl * 4 + sft) * P::RL4))[wd] = v;
-          }
-      } else if (step >= 1 && valid) {
-        // row `lane`, segment w - 4 = columns [2L (w-4), 2L (w-4) + 2L) -> L/2 limbs in place + a carry
-        const int sg = w - 4;
-        int32_t* crow = C + lane * P::CS + 2 * L * sg;
-        // all of the segment's column sums are read first (one LDS wait), then carried
+        for (int sft = 1; sft < 4; ++sft) dst[sft * (P::RL4 / 4)] = __builtin_amdgcn_alignbyte(r1, r0, (u32)sft);
+      }
+    };
+    // rounding of constant row s0's column sums (epilogue waves): thread = (row, segment), the 4 segments of a
+    // row in 4 neighbouring lanes of one wave.  Segment sg carries columns [2L sg, 2L sg + 2L) into L/2 limbs
+    // and a carry; the carries ripple up through the segments (lane shuffles), then the magnitude, its bit
+    // length and the rounding window (read from LDS at the row's bit offset) give norm(W) exactly as
+    // mp::norm<L, 2L + 2> does.
+    // value of lane k of this lane's quad (the 4 segments of a row), by DPP quad permutation
+    auto quad32 = [](u32 x, int k) -> u32 {
+      switch (k) {
+        case 0: return (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0x00, 0xF, 0xF, false);
+        case 1: return (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0x55, 0xF, 0xF, false);
+        case 2: return (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0xAA, 0xF, 0xF, false);
+        default: return (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0xFF, 0xF, 0xF, false);
+      }
+    };
+    auto quad64 = [&](i64 x, int k) -> i64 {
+      return (i64)(((u64)quad32((u32)((u64)x >> 32), k) << 32) | quad32((u32)x, k));
+    };
+    auto round_row = [&](int s0) {
+      const int et = tid - 256, er = et >> 2, sg = et & 3;
+      const i64 en = rb * 64 + er;
+      int32_t* crow = C + er * P::CS;
+      u32 v[P::SEGL];
+      i64 carry = 0;
+      {
         i64 part[P::SEGL];
 #pragma unroll
         for (int q = 0; q < P::SEGL; ++q) {
-          i64 v = 0;
+          i64 x = 0;
 #pragma unroll
-          for (int b = 0; b < 4; ++b) v += (i64)crow[4 * q + b] * ((i64)1 << (8 * b));
-          part[q] = v;
+          for (int b = 0; b < 4; ++b) x += (i64)crow[2 * L * sg + 4 * q + b] * ((i64)1 << (8 * b));
+          part[q] = x;
         }
-        i64 carry = 0;
 #pragma unroll
         for (int q = 0; q < P::SEGL; ++q) {
-          const i64 v = part[q] + carry;
-          crow[q] = (int32_t)(u32)v;
-          carry = v >> 32;
+          const i64 x = part[q] + carry;
+          v[q] = (u32)x;
+          carry = x >> 32;
         }
-        cseg[sg][lane] = carry;
       }
-      __syncthreads();
-      // phase B: math waves multiply row `step`; epilogue wave 4 rounds row step - 1
+      // carries up through the segments: segment k adds the final carry-out of segment k - 1
+      i64 cout = carry;   // this segment's carry-out once its carry-in is added
+#pragma unroll
+      for (int k = 1; k < 4; ++k) {
+        const i64 cin = quad64(cout, k - 1);
+        if (sg == k) {
+          const u32 ext = cin < 0 ? 0xFFFFFFFFu : 0u;
+          u32 cy = 0u;
+#pragma unroll
+          for (int q = 0; q < P::SEGL; ++q)
+            v[q] = mp::addc(v[q], q == 0 ? (u32)cin : (q == 1 ? (u32)((u64)cin >> 32) : ext), cy);
+          cout = carry + (i64)cy - (cin < 0 ? 1 : 0);
+        }
+      }
+      const i64 top = quad64(cout, 3);   // W's limbs 2L, 2L + 1
+      const bool neg = top < 0;
+      // magnitude: ~W + 1; the +1 reaches segment k when every lower segment of W is zero
+      u32 anyv = 0u;
+#pragma unroll
+      for (int q = 0; q < P::SEGL; ++q) anyv |= v[q];
+      const unsigned zmask = (anyv == 0u ? 1u : 0u) << sg;
+      const unsigned zall = quad32(zmask, 0) | quad32(zmask, 1) | quad32(zmask, 2) | quad32(zmask, 3);
+      const unsigned below = (1u << sg) - 1u;
+      if (neg) {
+        u32 cy = ((zall & below) == below) ? 1u : 0u;
+#pragma unroll
+        for (int q = 0; q < P::SEGL; ++q) v[q] = mp::addc(~v[q], 0u, cy);
+      }
+      u64 tmag = (u64)top;
+      if (neg) tmag = ~tmag + ((zall & 15u) == 15u ? 1u : 0u);
+      // bit length of |W|
+      int hb = 0;
+#pragma unroll
+      for (int q = 0; q < P::SEGL; ++q)
+        if (v[q]) hb = 32 * (sg * P::SEGL + q) + 32 - __builtin_clz(v[q]);
+      hb = max(max((int)quad32((u32)hb, 0), (int)quad32((u32)hb, 1)), max((int)quad32((u32)hb, 2), (int)quad32((u32)hb, 3)));
+      const int bl = tmag ? 64 * L + 64 - __builtin_clzll(tmag) : hb;
+      // |W| limbs into the row's LDS (its column sums are consumed), then one lane per row takes the window
+      u32* mrow = (u32*)crow;
+#pragma unroll
+      for (int q = 0; q < P::SEGL; ++q) mrow[sg * P::SEGL + q] = v[q];
+      if (sg == 3) {
+        mrow[2 * L] = (u32)tmag;
+        mrow[2 * L + 1] = (u32)(tmag >> 32);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (sg == 0 && en < SB) {
+        const i64 ea = Cex[s0], Er = Erow[er];
+        u32 out[L];
+        i64 e = B3D_EZ;
+        mp::zero(out);
+        if (bl != 0 && Er != B3D_EZ && ea != B3D_EZ) {
+          // floor(|W| / 2^(s - 1)), s = bl - NB (negative s: |W| 2^(1 - s), exact), NB + 1 bits in L limbs
+          int s = bl - (32 * L - 2);
+          const int sb = s - 1;
+          const int w0 = sb >> 5;                    // floor, also for negative sb
+          const u32 bs = (u32)(sb & 31);
+          u32 t[L];
+          u32 lo = (w0 >= 0 && w0 < 2 * L + 2) ? mrow[w0] : 0u;
+#pragma unroll
+          for (int k = 0; k < L; ++k) {
+            const int i = w0 + k + 1;
+            const u32 hi = (i >= 0 && i < 2 * L + 2) ? mrow[i] : 0u;
+            t[k] = b3d_alignbit(hi, lo, bs);
+            lo = hi;
+          }
+          u32 cy = 1u;
+#pragma unroll
+          for (int k = 0; k < L; ++k) t[k] = mp::addc(t[k], 0u, cy);
+#pragma unroll
+          for (int k = 0; k < L; ++k) out[k] = b3d_alignbit(k + 1 < L ? t[k + 1] : 0u, t[k], 1u);
+          const bool over = (out[L - 1] >> 30) != 0u;
+          if (over) {
+#pragma unroll
+            for (int k = 0; k < L; ++k) out[k] = k == L - 1 ? (1u << 29) : 0u;
+          }
+          s += over ? 1 : 0;
+          mp::cneg(out, neg ? 0xFFFFFFFFu : 0u);
+          e = ea + Er + s;
+        }
+        const i64 o = ob + s0 * SB + en;
+        mp::store(out, Om, ops, o);
+        Oe[o] = e;
+      }
+    };
+    __syncthreads();
+    if (math) build_copies(0);
+    __syncthreads();
+    for (int step = 0; step <= S; ++step) {
+      v4i acc[4][P::MT];
+      // phase 1: math waves multiply constant row `step`, epilogue waves round row step - 1
       if (math) {
         if (step < S) {
           const int32_t* cc = Cr + step * P::NCOL;
@@ -228,17 +350,36 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
             for (int rt = 0; rt < 4; ++rt) acc[rt][mt] = v4i{c0, c0, c0, c0};
           }
+          const int FGk = (S / 4) * P::CPJ;         // K-steps of whole operand groups
           for (int ks = 0; ks < nks; ++ks) {
-            const int q = 4 * ks + g;   // this lane's chunk: operand q / CPJ, bytes 16 (q % CPJ) ..
+            // this lane group's chunk (operand j, chunk ci), and the column tiles the K-step can reach
+            int j, ci, cmin, cmax;
+            if (ks < FGk) {
+              ci = ks % P::CPJ;
+              j = (ks / P::CPJ) * 4 + g;
+              cmin = ci;
+              cmax = ci + P::CPJ;
+            } else {
+              const int s0l = (ks - FGk) * 4, nleft = (S % 4) * P::CPJ;
+              const int slot = s0l + g;
+              j = (S / 4) * 4 + slot / P::CPJ;
+              ci = slot % P::CPJ;
+              // the K-step's slots run over ci = s0l % CPJ .. last % CPJ, wrapping to the next operand (ci 0)
+              // at most once
+              const int last = (s0l + 3 < nleft ? s0l + 3 : nleft - 1);
+              const bool wrap = (s0l / P::CPJ) != (last / P::CPJ);
+              cmin = wrap ? 0 : s0l % P::CPJ;
+              cmax = (wrap ? P::CPJ - 1 : last % P::CPJ) + P::CPJ;
+            }
             v4i A[4];
 #pragma unroll
-            for (int rt = 0; rt < 4; ++rt) A[rt] = *(const v4i*)(X + (16 * rt + r) * P::XROW + 16 * q);
-            const int jl = q / P::CPJ;
-            const int i0 = 16 * (q - jl * P::CPJ);
-            // padding chunks past the operands: B fragment 0 (loads from a valid operand, masked)
-            const int lmask = jl < S ? -1 : 0;
-            const int jc = jl < S ? jl : S - 1;
-            // every fragment of the K-step is read before the products, one LDS wait per K-step
+            for (int rt = 0; rt < 4; ++rt) A[rt] = *(const v4i*)(X + (16 * rt + r) * P::XROW + (ks * 4 + g) * 16);
+            const int i0 = 16 * ci;
+            // lane groups past the operands: B fragment 0 (loads from a valid operand, masked)
+            const int lmask = j < S ? -1 : 0;
+            const int jc = j < S ? j : S - 1;
+            // every fragment of the K-step is read first (one LDS wait per K-step); column tiles the K-step
+            // cannot reach hold zero fragments and skip their products (a wave-uniform test)
             v4i Bf[P::MT];
 #pragma unroll
             for (int mt = 0; mt < P::MT; ++mt) {
@@ -250,46 +391,22 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void k
               Bf[mt] = v4i{(int)src[0] & lmask, (int)src[1] & lmask, (int)src[2] & lmask, (int)src[3] & lmask};
             }
 #pragma unroll
-            for (int mt = 0; mt < P::MT; ++mt)
+            for (int mt = 0; mt < P::MT; ++mt) {
+              const int ct = w + 4 * mt;
+              if (ct < P::NCT && ct >= cmin && ct <= cmax) {
 #pragma unroll
-              for (int rt = 0; rt < 4; ++rt)
-                acc[rt][mt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[rt], Bf[mt], acc[rt][mt], 0, 0, 0);
+                for (int rt = 0; rt < 4; ++rt)
+                  acc[rt][mt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[rt], Bf[mt], acc[rt][mt], 0, 0, 0);
+              }
+            }
           }
         }
-      } else if (w == 4 && step >= 1 && valid) {
-        const int s0 = step - 1;
-        u32 W[2 * L + 2];
-#pragma unroll
-        for (int sg = 0; sg < 4; ++sg)
-#pragma unroll
-          for (int q = 0; q < P::SEGL; ++q) W[sg * P::SEGL + q] = (u32)C[lane * P::CS + 2 * L * sg + q];
-        const i64 c3 = cseg[3][lane];
-        W[2 * L] = (u32)c3;
-        W[2 * L + 1] = (u32)((u64)c3 >> 32);
-#pragma unroll
-        for (int sg = 0; sg < 3; ++sg) {
-          const i64 cs = cseg[sg][lane];
-          const u32 ext = cs < 0 ? 0xFFFFFFFFu : 0u;
-          u32 cy = 0u;
-#pragma unroll
-          for (int q = (sg + 1) * P::SEGL; q < 2 * L + 2; ++q)
-            W[q] = mp::addc(W[q], q == (sg + 1) * P::SEGL ? (u32)cs : (q == (sg + 1) * P::SEGL + 1 ? (u32)((u64)cs >> 32) : ext), cy);
-        }
-        const i64 ea = Cex[s0], Er = Erow[lane];
-        u32 out[L];
-        i64 e;
-        if (Er == B3D_EZ || ea == B3D_EZ) {
-          mp::zero(out);
-          e = B3D_EZ;
-        } else {
-          e = mp::norm<L, 2 * L + 2>(W, ea + Er, out);
-        }
-        const i64 o = ob + s0 * SB + n;
-        mp::store(out, Om, ops, o);
-        Oe[o] = e;
+      } else if (step >= 1) {
+        round_row(step - 1);
       }
       __syncthreads();
-      // phase C: column sums of row `step` into LDS (the epilogue has read row step - 1's)
+      // phase 2: column sums of row `step` into LDS (the epilogue has read row step - 1's), then the
+      // reversed digit copies of row step + 1 (the products of row `step` are done)
       if (math && step < S) {
 #pragma unroll
         for (int rt = 0; rt < 4; ++rt)
@@ -300,6 +417,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
               for (int i = 0; i < 4; ++i) C[(16 * rt + 4 * g + i) * P::CS + 16 * ct + r] = acc[rt][mt][i];
           }
+        if (step + 1 < S) build_copies(step + 1);
       }
       __syncthreads();
     }

@@ -2,7 +2,9 @@
 CPU with numpy: balanced base-256 digits of the constants, the Toeplitz column sums of the two's complement
 bytes fed as u ^ 0x80 with the per-column correction, the carry pass in four segments and their assembly.
 Checked against the exact integer sum of products and, after rounding, against the oracle's
-gemm_aligned (the VALU path's rounding).  CPU only."""
+gemm_aligned (the VALU path's rounding); the kernel's segmented rounding (segment carries rippled up, the
+magnitude from per-segment zero flags, the bit length, the window at the row's bit offset) against the
+oracle's norm.  CPU only."""
 import random
 
 import numpy as np
@@ -72,6 +74,54 @@ def carry_segments(C, corr, L):
     return W
 
 
+def segmented_round(C, corr, L, e):
+    """the kernel's rounding of one row's column sums (round_row in b3d_pole.hip), restated"""
+    segl, M32 = L // 2, (1 << 32) - 1
+    v, carry = [], []
+    for s in range(4):
+        c, limbs = 0, []
+        for q in range(segl):
+            x = c + sum((int(C[2 * L * s + 4 * q + b]) + int(corr[2 * L * s + 4 * q + b])) << (8 * b) for b in range(4))
+            limbs.append(x & M32)
+            c = x >> 32
+        v.append(limbs)
+        carry.append(c)
+    cout = [carry[0]]
+    for k in range(1, 4):
+        cin = cout[k - 1]
+        U = sum(x << (32 * q) for q, x in enumerate(v[k])) + (cin & ((1 << (32 * segl)) - 1) if cin >= 0 else
+                                                              (1 << (32 * segl)) + cin)
+        cy = U >> (32 * segl)
+        v[k] = [(U >> (32 * q)) & M32 for q in range(segl)]
+        cout.append(carry[k] + cy - (1 if cin < 0 else 0))
+    top = cout[3]
+    neg = top < 0
+    zero = [all(x == 0 for x in seg) for seg in v]
+    if neg:
+        for k in range(4):
+            cy = 1 if all(zero[:k]) else 0
+            U = sum((~x & M32) << (32 * q) for q, x in enumerate(v[k])) + cy
+            v[k] = [(U >> (32 * q)) & M32 for q in range(segl)]
+        tmag = (~top & ((1 << 64) - 1)) + (1 if all(zero) else 0)
+    else:
+        tmag = top
+    mag = [x for seg in v for x in seg] + [tmag & M32, (tmag >> 32) & M32]
+    A = sum(x << (32 * k) for k, x in enumerate(mag))
+    bl = A.bit_length()
+    if bl == 0:
+        return 0, R.EZ
+    s = bl - R.NB(L)
+    sb = s - 1
+    w0, bs = sb >> 5, sb & 31
+    get = lambda i: mag[i] if 0 <= i < 2 * L + 2 else 0
+    t = [((get(w0 + k + 1) << 32 | get(w0 + k)) >> bs) & M32 for k in range(L)]
+    T = sum(x << (32 * k) for k, x in enumerate(t)) + 1
+    m = (T >> 1) & ((1 << (32 * L)) - 1)
+    if m >> (32 * L - 2):
+        m, s = 1 << (32 * L - 3), s + 1
+    return (-m if neg else m), e + s
+
+
 @pytest.mark.parametrize("L", [4, 6, 10, 34])
 def test_pole_digits_exact(L):
     rng = random.Random(900 + L)
@@ -97,3 +147,24 @@ def test_pole_digits_exact(L):
         assert abs(exact) < 1 << (32 * (2 * L + 2) - 1)
         if ea != R.EZ and xe != R.EZ:
             assert R.norm(exact, ea + xe, L) == R.gemm_aligned(am, ea, xm, xe, L), trial
+            assert segmented_round(C, corr, L, ea + xe) == R.norm(exact, ea + xe, L), trial
+
+
+@pytest.mark.parametrize("L", [4, 10, 34])
+def test_segmented_round_edges(L):
+    """column sums of chosen integers W (base-256 digits, signed top): cancellation to tiny magnitudes,
+    negative values whose low segments are zero, powers of two and values just below them"""
+    rng = random.Random(950 + L)
+    nb = R.NB(L)
+    segbits = 32 * (L // 2)
+    cases = [1, -1, 2, -2, 3, (1 << nb) - 1, -((1 << nb) - 1), 1 << nb, -(1 << nb), (1 << (2 * nb)) + 1,
+             -(1 << (2 * nb)), -(1 << segbits), -(1 << (2 * segbits)), -(3 << (3 * segbits)), 1 << (3 * segbits),
+             (1 << (3 * segbits)) - 1, -((1 << (2 * nb + 3)) - 1)]
+    cases += [rng.choice((-1, 1)) * rng.getrandbits(rng.randint(1, 2 * nb + 3)) for _ in range(200)]
+    for W in cases:
+        assert abs(W) < 1 << (64 * L + 60)
+        u = W % (1 << (64 * L))                        # low 8L digits; the rest goes to the top column
+        C = np.array([(u >> (8 * c)) & 0xFF for c in range(8 * L)], dtype=np.int64)
+        C[8 * L - 1] += (W - u) >> (8 * (8 * L - 1))   # top column carries the signed remainder
+        got = segmented_round(C, np.zeros(8 * L, dtype=np.int64), L, 77)
+        assert got == R.norm(W, 77, L), W
