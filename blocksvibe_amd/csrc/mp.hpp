@@ -459,10 +459,15 @@ __device__ __forceinline__ void mac28(int32_t (&acc)[2 * NL], const int32_t (&a)
   i64 carry = 0;
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
-    i64 s = (i64)acc[c] + carry;
+    // two independent partial sums (alternate terms): two multiply-add chains in flight per column
+    i64 s0 = (i64)acc[c] + carry, s1 = 0;
+    const int lo = c - NL + 1 > 0 ? c - NL + 1 : 0, hi = c < NL - 1 ? c : NL - 1;
 #pragma unroll
-    for (int i = (c - NL + 1 > 0 ? c - NL + 1 : 0); i <= (c < NL - 1 ? c : NL - 1); ++i)
-      s += (i64)a[i] * (i64)y[c - i];
+    for (int i = lo; i <= hi; ++i) {
+      if ((i - lo) & 1) s1 += (i64)a[i] * (i64)y[c - i];
+      else s0 += (i64)a[i] * (i64)y[c - i];
+    }
+    const i64 s = s0 + s1;
     acc[c] = (int32_t)(s & 0x0FFFFFFF);
     carry = s >> 28;
   }
