@@ -24,6 +24,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 ORDER, LAMBDA, KEPT, PREC, TJMAX, D12 = 60, 11, 14, 1024, 60, "6"
+WORKLOAD = ("nmax=6 spin-2 externals: 25 operator groups x 22 structures x 6 lambda orders, order 60, kept 14, "
+            "1024 bits, xt")
 TWO_JS = (4, 4, 4, 4)
 NQ_PER_SIGN = 11
 
@@ -211,11 +213,11 @@ def pmc_traffic(workload):
     (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by its reading for a plain copy of known size),
     or None when there is no summary for this workload"""
     try:
-        kv = dict(line.split("=", 1) for line in open(PMC_SUMMARY).read().splitlines()
-                  if "=" in line and not line.startswith("#"))
+        kv = {k.strip(): v.strip() for k, v in (line.split("=", 1) for line in open(PMC_SUMMARY).read().splitlines()
+                                                if "=" in line and not line.startswith("#"))}
     except OSError:
         return None
-    if kv.get("workload", "").strip() != workload:
+    if kv.get("workload", "") != workload or "traffic_bytes_per_launch" not in kv:
         return None
     return float(kv["traffic_bytes_per_launch"])
 
@@ -313,8 +315,7 @@ def main():
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32 limbs (1086-bit mpf)",
            "data": "synthetic (seeded random M matrices and h_inf of the real shapes)",
-           "config": {"workload": "nmax=6 spin-2 externals: 25 operator groups x 22 structures x 6 lambda orders, "
-                                  "order 60, kept 14, 1024 bits, xt",
+           "config": {"workload": WORKLOAD if args.groups == "all" else f"{WORKLOAD}; groups {args.groups} only",
                       "groups": len(runs), "tasks": 22 * len(runs), "blocks_per_step": blocks // args.steps}}
     res["roofline"] = roofline(timer, res["config"]["workload"])
     print(f"# roofline: {res['roofline']}", file=sys.stderr, flush=True)
