@@ -445,49 +445,41 @@ def gpu_group(tasks, q4s, sfile, mfile, timers):
     return results
 
 
-_JOBS = {}
-
-
 def write_outputs(tasks, results):
-    """all spin files of all tasks, formatted in parallel worker processes (host data only)"""
-    import multiprocessing as mpr
+    """all spin files of all tasks (host data only): numbers formatted by the library's host formatter on up
+    to num_threads threads, one spin file after another"""
     jobs = [(i, tj) for i, t in enumerate(tasks) for tj in t.two_js_internal]
-    _JOBS.update(tasks=tasks, results=results)
-    nproc = max(1, min(len(jobs), max(t.num_threads for t in tasks), os.cpu_count() or 1))
-    try:
-        if nproc == 1:
-            for j in jobs:
-                _write_one(j)
-        else:
-            with mpr.get_context("fork").Pool(nproc) as pool:
-                for _ in pool.imap_unordered(_write_one, jobs):
-                    pass
-    finally:
-        _JOBS.clear()
+    threads = max(1, min(max(t.num_threads for t in tasks), os.cpu_count() or 1))
+    for j in jobs:
+        _write_one(tasks, results, j, threads)
 
 
-def _write_one(job):
+def _write_one(tasks, results, job, threads):
+    import numpy as np
+
     from . import b3djson as BJ
     from . import structure as S
-    from .mpf import to_pairs
     i, tj = job
-    t = _JOBS["tasks"][i]
-    tjs, pp, vals, outs, Sm = _JOBS["results"][(tuple(t.q4), t.sign)]
+    t = tasks[i]
+    tjs, pp, vals, outs, Sm = results[(tuple(t.q4), t.sign)]
     jo = tjs.index(tj)
+    p = BJ.digits10(t.precision)
     structs = [(len(S.so3(pp[k][0], t.two_j12, tj)), len(S.so3(pp[k][1], t.two_j43, tj))) for k in range(2)]
     coord_rows = []
     for c in [c for c in BJ.COORD_ORDER if c in t.coordinates]:
         m, e = vals[(c, t.delta_1_plus_2)]                          # (L, Jo, 2, S2, outputs, Dmax)
         outputs = outs[(c, t.delta_1_plus_2)]
         mj, ej = m[:, jo], e[jo]
-        shape = ej.shape                                            # (2, S2, outputs, Dmax)
-        flat = to_pairs(mj, ej)
-        no, nd = shape[2], shape[3]
-
-        def poly(k, s, o):
-            base = ((k * shape[1] + s) * no + o) * nd
-            return flat[base:base + nd]
-
+        _, S2, no, _ = ej.shape                                     # (2, S2, outputs, Dmax)
+        sel, where = [], {}
+        for k in range(2):
+            for a in range(structs[k][0]):
+                for b in range(structs[k][1]):
+                    for o in range(no):
+                        where[(k, a, b, o)] = len(sel)
+                        sel.append((k * S2 + a * Sm + b) * no + o)
+        bodies = BJ.poly_bodies(mj, ej, p, BJ.trim_exponent(p), select=np.array(sel, dtype=np.int64),
+                                threads=threads)
         rows_k = []
         for k in range(2):
             rows_a = []
@@ -496,7 +488,7 @@ def _write_one(job):
                 for b in range(structs[k][1]):
                     rows = {}
                     for o, (n, _) in enumerate(outputs):
-                        rows.setdefault(n, []).append(poly(k, a * Sm + b, o))
+                        rows.setdefault(n, []).append(bodies[where[(k, a, b, o)]])
                     rows_b.append([rows[n] for n in sorted(rows)])
                 rows_a.append(rows_b)
             rows_k.append(rows_a)

@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-#define B3D_ABI_VERSION 6
+#define B3D_ABI_VERSION 7
 
 /* Last error message of the calling thread (empty if none). */
 const char* b3d_last_error(void);
@@ -148,6 +148,29 @@ int b3d_rns_gemm(int L, int nmod, const uint16_t* primes, const int8_t* w8, cons
                  const int32_t* Bm, const int64_t* Be, int64_t bps, int64_t bsb, int64_t bsk, int64_t bsj, int32_t* Cm,
                  int64_t* Ce, int64_t cps, int64_t csb, int64_t csi, int64_t csj, int64_t Nc, int tile_wm,
                  int64_t* EB, void* workspace, int64_t ws_bytes, void* stream);
+
+/* ---- output emission (host code; host arrays; no GPU needed) ---- */
+
+/* Decimal text of n MPF entries read from host limb planes (entry i: limbs m[k * ps + i], exponent e[i]): the
+ * exact value rounded to p significant digits, ties to even, trailing zeros removed, fixed notation for
+ * decimal exponents -4 <= X < p, else d.ddde+XX; zero as "0" (blocks_3d's mpf_float output at stream
+ * precision p; replaces reference blocksvibe/b3djson.py:30-68 fmt_big / fmt_gmpy, which convert each value to
+ * an mpmath or MPFR number first).  The texts are concatenated into out (cap bytes, no terminators); text i
+ * ends at end[i].  threads: worker threads (<= 0: one per hardware thread).  1 <= L <= 64, 1 <= p <= 2000. */
+int b3d_format_numbers(int L, const int32_t* m, const int64_t* e, int64_t ps, int64_t n, int p, char* out,
+                       int64_t cap, int64_t* end, int threads);
+/* Kept length of each of npoly polynomials of deg coefficients (coefficient d of polynomial q is entry
+ * q * deg + d of host limb planes m / exponents e): one past the last coefficient c with
+ * max(|later coefficients|) < 2^eps_exp |c|, by exact comparison; 0 if every coefficient is zero (reference
+ * blocksvibe/b3djson.py:160-168, blocks_3d's dropping of negligible top coefficients). */
+int b3d_poly_sizes(int L, const int32_t* m, const int64_t* e, int64_t ps, int64_t npoly, int deg, int64_t eps_exp,
+                   int32_t* sizes);
+/* Both at once for npoly polynomials laid out as in b3d_poly_sizes: each polynomial's JSON list body as
+ * blocks_3d's write_derivs.cxx writes it (reference blocksvibe/b3djson.py:150-177): the kept coefficients
+ * quoted and joined by ",\n" ("c0",\n"c1"), or "0" (quoted) if every coefficient is zero.  Bodies are
+ * concatenated into out (cap bytes); body q ends at end[q].  Work is split over threads by polynomial. */
+int b3d_format_polys(int L, const int32_t* m, const int64_t* e, int64_t ps, int64_t npoly, int deg, int p,
+                     int64_t eps_exp, char* out, int64_t cap, int64_t* end, int threads);
 
 #ifdef __cplusplus
 }
