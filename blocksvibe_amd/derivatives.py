@@ -7,7 +7,7 @@ Per (spin j, parity, structure), with w[k][n] = n^k r_c^n, r_c = 3 - 2 sqrt 2, R
 poles i of spin j at full order (positions D_i, shifts s_i), x = Delta - ub_j:
   c[k]    = sum_{m <= order} w[k][m] h_inf[m]                       (one GEMM, K = order + 1)
   v_i[k]  = sum_m w[k][m + s_i] res[m][i]                           (accumulated level by level)
-  a_i[e]  = sum_{d <= e} C(e, d) D_i^(e - d) v_i[d]                 (batched (R+1)^2 GEMMs)
+  a_i[e]  = sum_{d <= e} C(e, d) D_i^(e - d) v_i[d]                 (triangular products, exact integers)
   G[w][d] = sum_i D_i^w v_i[d];  H[e][s] = sum_d C(e, d) G[s-1-d][d] (GEMMs)
   P_e[t]  = C(e, t) c[e - t] + H[e][e - t]
   numerator_e(x) = P_e(x + ub) Pi(x) + sum_i a_i[e] Pi_i(x)          (one GEMM with an exact matrix)
@@ -189,6 +189,7 @@ class Derivs:
                 for k in range(R + 1):
                     pairs.append(mpf.norm_int(comb(e, k) * d ** (e - k), k - e, L) if k <= e else (0, mpf.EZ))
         self.Ttab = mpf.from_pairs(pairs, L, (len(d2) + 1, R + 1, R + 1))
+        self.tri = self._tri_tables(d2, R + 1)
         didx = np.zeros((Jo, NP), dtype=np.int64)
         for jo, (tj, ps) in enumerate(zip(self.tjs, self.poles)):
             didx[jo, :len(ps)] = [di[int(S.pole_delta2(t, k, tj))] for t, k in ps]
@@ -287,7 +288,20 @@ class Derivs:
             J = j1 - j0
             v = mom[j0:j1].reshape(J * NP, R1, Nc)
             T = self.Ttab.take(self.didx[j0:j1].reshape(-1))                            # (J NP, e, d)
-            a = rns.gemm(T, v)                                                          # (J NP, e, Nc)
+            # numerator GEMM operand (J, NP + R1, e Nc): rows < NP the residues a_i[e], then P below
+            Bn = mpf.MPF(torch.empty((L, J, NP + R1, R1 * Nc), dtype=torch.int32, device=v.e.device),
+                         torch.empty((J, NP + R1, R1 * Nc), dtype=torch.int64, device=v.e.device))
+            if self.tri is not None:
+                TL, tm, tsg, texp = self.tri
+                v = v.contiguous()
+                _lib.call("b3d_tri_product", L, TL, v.m.data_ptr(), v.e.data_ptr(), v.numel(), J * NP, R1, Nc,
+                          self.didx[j0:j1].contiguous().data_ptr(), tm.data_ptr(), tsg.data_ptr(), tsg.numel(),
+                          texp.data_ptr(), Bn.m.data_ptr(), Bn.e.data_ptr(), Bn.numel(), NP, (NP + R1) * R1 * Nc,
+                          R1 * Nc, mpf.stream_ptr())
+            else:
+                a = rns.gemm(T, v)                                                      # (J NP, e, Nc)
+                Bn.m[:, :, :NP] = a.m.reshape(L, J, NP, R1 * Nc)
+                Bn.e[:, :NP] = a.e.reshape(J, NP, R1 * Nc)
             Vand = mpf.MPF(T.m[:, :, :Rw, 0].reshape(L, J, NP, Rw).permute(0, 1, 3, 2).contiguous(),
                            T.e[:, :Rw, 0].reshape(J, NP, Rw).permute(0, 2, 1).contiguous())  # (J, w, NP)
             G = rns.gemm(Vand, v.reshape(J, NP, R1 * Nc))                               # (J, w, d Nc)
@@ -300,11 +314,32 @@ class Derivs:
             cbx = mpf.MPF(cb.m.expand(L, R1 * R1, J, Nc).contiguous(), cb.e.expand(R1 * R1, J, Nc).contiguous())
             P = mpf.add(mpf.mul(cz.take(cidx), cbx), Hz.take(hidx))                     # ((t, e), J, Nc)
             P = P.reshape(R1, R1, J, Nc).permute(2, 0, 1, 3).reshape(J, R1, R1 * Nc)    # (J, t, e Nc)
-            Bn = mpf.cat([a.reshape(J, NP, R1 * Nc), P], 1)                             # (J, NP + R1, e Nc)
+            Bn.m[:, :, NP:] = P.m
+            Bn.e[:, NP:] = P.e
             A = self.Anum[j0:j1]
             num = rns.gemm(A.contiguous(), Bn)                                          # (J, Dout, e Nc)
             outs.append(num.reshape(J, self.Dout, R1, 2, S2, B))
         return outs[0] if len(outs) == 1 else mpf.cat(outs, 0)
+
+    @staticmethod
+    def _tri_tables(d2, R1):
+        """the same matrices C(e, k) D^(e - k) as exact integers C(e, k) d^(e - k) 2^k (d = 2 D) with the row
+        unit 2^-e, for b3d_tri_product: (TL, magnitude limb planes (TL, ntab, R1, R1), sign masks
+        (ntab, R1, R1), unit exponents (ntab, R1)); None if an entry needs more than 8 limbs"""
+        ints = [[0] * (R1 * R1)] + [[comb(e, k) * d ** (e - k) * 2 ** k if k <= e else 0
+                                     for e in range(R1) for k in range(R1)] for d in d2]
+        bits = max(abs(x).bit_length() for row in ints for x in row)
+        TL = next((t for t in (2, 4, 8) if bits <= 32 * t), None)
+        if TL is None:
+            return None
+        mags = np.array([abs(x) for row in ints for x in row], dtype=object)
+        tm = np.array([[(int(x) >> (32 * l)) & 0xFFFFFFFF for x in mags] for l in range(TL)], dtype=np.uint64)
+        tm = torch.from_numpy(tm.astype(np.uint32).view(np.int32).reshape(TL, len(ints), R1, R1)).to(mpf.dev())
+        tsg = torch.tensor([-1 if x < 0 else 0 for row in ints for x in row], dtype=torch.int32,
+                           device=mpf.dev()).reshape(len(ints), R1, R1)
+        texp = torch.tensor([-e for _ in ints for e in range(R1)], dtype=torch.int64,
+                            device=mpf.dev()).reshape(len(ints), R1)
+        return TL, tm, tsg, texp
 
     def _finish_consts(self):
         """index tables and binomials of finish() for this R (device constants, built once so that finish()
@@ -336,6 +371,9 @@ class Derivs:
         A = self.Anum
         d.Anum = mpf.MPF(A.m[:, :, :d.Dout, :W].contiguous(), A.e[:, :d.Dout, :W].contiguous())
         d.Ttab = mpf.MPF(self.Ttab.m[:, :, :R1, :R1].contiguous(), self.Ttab.e[:, :R1, :R1].contiguous())
+        if self.tri is not None:
+            TL, tm, tsg, texp = self.tri
+            d.tri = (TL, tm[:, :, :R1, :R1].contiguous(), tsg[:, :R1, :R1].contiguous(), texp[:, :R1].contiguous())
         d.Wh = mpf.MPF(self.Wh.m[:, :R1].contiguous(), self.Wh.e[:R1].contiguous())
         d.Cb = mpf.MPF(self.Cb.m[:, :R1, :R1].contiguous(), self.Cb.e[:R1, :R1].contiguous())
         v = [n ** k for n in range(self.NK.shape[1] // (self.R + 1)) for k in range(R1)]

@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-#define B3D_ABI_VERSION 7
+#define B3D_ABI_VERSION 8
 
 /* Last error message of the calling thread (empty if none). */
 const char* b3d_last_error(void);
@@ -123,6 +123,18 @@ int b3d_moments_stage(int L, const int32_t* rm, const int64_t* re, int64_t rps, 
 int b3d_moments_flush(int L, int KL, const int32_t* sm, const int64_t* se, int64_t sps, int W, int m0,
                       const int32_t* srow, const uint32_t* NK, int64_t nkps, const int32_t* NKb, int32_t* vm,
                       int64_t* ve, int64_t vps, int64_t nrows, int64_t N, int R1, void* stream);
+
+/* Lower-triangular products by exact integer matrices, in the multiprecision GEMM's semantics (column of
+ * v aligned to its largest exponent with truncated magnitudes, exact sum, one rounding; the same bits as
+ * b3d_rns_gemm with the matrix as A): out(b, e, c) = sum_{k <= e} T_t[e][k] v(b, k, c), t = tsel[b], for
+ * b < nb, e < K, c < Nc.  v entry (b, k, c) at (b K + k) Nc + c (plane stride vps).  T_t[e][k] = +-tm 2^texp:
+ * magnitudes as TL (2, 4 or 8) limb planes tm (ntab, K, K) (plane stride tps), sign masks tsg (ntab, K, K),
+ * unit exponents texp (ntab, K).  Output entry (b, e, c) at (b / nper) obatch + (b % nper) orow + e Nc + c
+ * (plane stride ops).  Replaces the batched GEMM a_i = C(e, k) D_i^(e-k) v_i of reference
+ * blocksvibe/derivatives.py:553-554 (one conversion to residues per v entry and one CRT per output there). */
+int b3d_tri_product(int L, int TL, const int32_t* vm, const int64_t* ve, int64_t vps, int64_t nb, int K, int64_t Nc,
+                    const int64_t* tsel, const int32_t* tm, const uint32_t* tsg, int64_t tps, const int64_t* texp,
+                    int32_t* om, int64_t* oe, int64_t ops, int64_t nper, int64_t obatch, int64_t orow, void* stream);
 
 /* ---- exact multiprecision GEMM through residues on int8 MFMA (replaces the unvendored sdpvibe.mpf.gemm
  *      reached through reference blocksvibe/recursion_gpu.py:53-79 gemm/bgemm, called at

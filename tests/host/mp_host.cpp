@@ -118,6 +118,23 @@ static int64_t mom_t(const int32_t* sm, const int64_t* se, int64_t sps, int W, i
   return e;
 }
 
+// tri_entry on host arrays: v (L planes of n entries, exponents ve), table (TL planes, signs), one output
+template <int L, int TL> static int64_t tri_t(const int32_t* vm, const int64_t* ve, int64_t vps, int64_t vbase,
+                                              int64_t vstride, int K, int e, const int32_t* tm, const uint32_t* tsg,
+                                              int64_t tps, int64_t tb, int64_t texp, uint32_t* r) {
+  u32 R[L];
+  const int64_t eo = mp::tri_entry<L, TL>(vm, ve, vps, vbase, vstride, K, e, tm, tsg, tps, tb, texp, R);
+  copy_out(r, R);
+  return eo;
+}
+template <int L> static int64_t tri_tl(int TL, const int32_t* vm, const int64_t* ve, int64_t vps, int64_t vbase,
+                                       int64_t vstride, int K, int e, const int32_t* tm, const uint32_t* tsg,
+                                       int64_t tps, int64_t tb, int64_t texp, uint32_t* r) {
+  if (TL == 2) return tri_t<L, 2>(vm, ve, vps, vbase, vstride, K, e, tm, tsg, tps, tb, texp, r);
+  if (TL == 4) return tri_t<L, 4>(vm, ve, vps, vbase, vstride, K, e, tm, tsg, tps, tb, texp, r);
+  return tri_t<L, 8>(vm, ve, vps, vbase, vstride, K, e, tm, tsg, tps, tb, texp, r);
+}
+
 extern "C" {
 int64_t hc_add(int L, const uint32_t* a, int64_t ea, const uint32_t* b, int64_t eb, uint32_t* r) {
   DISP(L, add_t<LL>(a, ea, b, eb, r))
@@ -141,5 +158,9 @@ int64_t hc_mom(int L, const int32_t* sm, const int64_t* se, int64_t sps, int W, 
                int64_t c, int n0, int k, int R1, const uint32_t* NK, int64_t nkps, int KL, const int32_t* NKb,
                uint32_t* r) {
   DISP(L, mom_t<LL>(sm, se, sps, W, nrows, N, row, c, n0, k, R1, NK, nkps, KL, NKb, r))
+}
+int64_t hc_tri(int L, int TL, const int32_t* vm, const int64_t* ve, int64_t vps, int64_t vbase, int64_t vstride, int K,
+               int e, const int32_t* tm, const uint32_t* tsg, int64_t tps, int64_t tb, int64_t texp, uint32_t* r) {
+  DISP(L, tri_tl<LL>(TL, vm, ve, vps, vbase, vstride, K, e, tm, tsg, tps, tb, texp, r))
 }
 }

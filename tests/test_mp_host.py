@@ -1,7 +1,8 @@
 """The gfx950 kernels' per-thread multiprecision helpers (blocksvibe_amd/csrc/mp.hpp), compiled for the
 host CPU (tests/host/mp_host.cpp), against the oracle bit for bit: add, mul (28-bit limb products), norm at
 every accumulator width the kernels use, the FP64 one-limb division (every digit exact, divisors up to
-2^15 - 1), barrel shifts (logical and arithmetic), Cauchy entries with h_inf, pole-step dot products and moment windows.  CPU only;
+2^15 - 1), barrel shifts (logical and arithmetic), Cauchy entries with h_inf, pole-step dot products, moment
+windows and the derivative stage's triangular products by exact integer matrices.  CPU only;
 the same source is what the GPU runs."""
 import ctypes
 import os
@@ -28,7 +29,8 @@ def lib():
                     os.path.join(HERE, "host", "mp_host.cpp")], check=True)
     os.replace(tmp, SO)
     lb = ctypes.CDLL(SO)
-    for f in (lb.hc_add, lb.hc_mul, lb.hc_norm, lb.hc_div, lb.hc_shift, lb.hc_cauchy, lb.hc_dot, lb.hc_mom):
+    for f in (lb.hc_add, lb.hc_mul, lb.hc_norm, lb.hc_div, lb.hc_shift, lb.hc_cauchy, lb.hc_dot, lb.hc_mom,
+              lb.hc_tri):
         f.restype = I64
     return lb
 
@@ -203,3 +205,43 @@ def test_mom_window(lib, L):
         ys = [vals[(w * nrows + row) * N + c] for w in range(W)]
         nks = [(n0 + w) ** k for w in range(W)]
         assert got(r, e) == R.mom_flush(ys, nks, (0, R.EZ), L), trial
+
+
+def tri_tables(ds, K):
+    """the derivative stage's matrices C(e, k) D^(e-k), D = d / 2, as exact integers C(e, k) d^(e-k) 2^k with
+    the row unit 2^-e: table 0 all zero (padding), then one per d"""
+    from math import comb
+    ints = [[0] * (K * K)] + [[comb(e, k) * d ** (e - k) * 2 ** k if k <= e else 0 for e in range(K) for k in range(K)]
+                              for d in ds]
+    return ints
+
+
+@pytest.mark.parametrize("L", [4, 10, 34, 50])
+def test_tri_entry(lib, L):
+    """mp::tri_entry against the oracle's GEMM (row of C(e, k) D^(e-k) as normalised numbers, column of v)"""
+    from fractions import Fraction
+    rng = random.Random(700 + L)
+    for trial in range(150):
+        K = rng.randint(1, 20 if L >= 10 else 12)
+        ds = [rng.choice([0, 1, -1, rng.randint(-320, 320)]) for _ in range(3)]
+        ints = tri_tables(ds, K)
+        bits = max(abs(x).bit_length() for row in ints for x in row)
+        TL = next(t for t in (2, 4, 8) if bits <= 32 * t)
+        ntab = len(ints)
+        mags = [abs(x) for row in ints for x in row]
+        tm = np.array([[(x >> (32 * l)) & 0xFFFFFFFF for x in mags] for l in range(TL)], dtype=np.uint32).view(np.int32)
+        tsg = np.array([0xFFFFFFFF if x < 0 else 0 for row in ints for x in row], dtype=np.uint32)
+        spread = rng.choice([2, 40, 2000])
+        ncol = 3
+        vals = [rp(rng, L, spread=spread) if rng.random() < 0.85 else (0, R.EZ) for _ in range(K * ncol)]
+        if trial % 10 == 0:
+            vals = [(0, R.EZ)] * (K * ncol)                      # all-zero columns
+        vm = np.stack([limbs(v[0], L) for v in vals], 1).view(np.int32).copy()
+        ve = np.array([v[1] for v in vals], np.int64)
+        t, e, c = rng.randrange(ntab), rng.randrange(K), rng.randrange(ncol)
+        r = np.zeros(L, np.uint32)
+        ex = lib.hc_tri(L, TL, ptr(vm), ptr(ve), I64(len(vals)), I64(c), I64(ncol), K, e, ptr(tm), ptr(tsg),
+                        I64(len(mags)), I64((t * K + e) * K), I64(-e), ptr(r))
+        arow = [R.from_fraction(Fraction(ints[t][e * K + k], 2 ** e), L) for k in range(K)]
+        col = [[vals[k * ncol + c]] for k in range(K)]
+        assert got(r, ex) == R.gemm([arow], col, L)[0][0], (trial, t, e, c)
