@@ -474,6 +474,28 @@ __device__ __forceinline__ void mac28(int32_t (&acc)[2 * NL], const int32_t (&a)
   acc[NC] += (int32_t)carry;
 }
 
+// The columns c >= C0 of mac28 only (their partial products i + k = c), with no carry into column C0: acc
+// then holds the upper part of the product, short of the carry out of the columns below C0.
+template <int C0, int NL>
+__device__ __forceinline__ void mac28_from(int32_t (&acc)[2 * NL], const int32_t (&a)[NL], const int32_t (&y)[NL]) {
+  constexpr int NC = 2 * NL - 1;
+  i64 carry = 0;
+#pragma unroll
+  for (int c = C0; c < NC; ++c) {
+    i64 s0 = (i64)acc[c] + carry, s1 = 0;
+    const int lo = c - NL + 1 > 0 ? c - NL + 1 : 0, hi = c < NL - 1 ? c : NL - 1;
+#pragma unroll
+    for (int i = lo; i <= hi; ++i) {
+      if ((i - lo) & 1) s1 += (i64)a[i] * (i64)y[c - i];
+      else s0 += (i64)a[i] * (i64)y[c - i];
+    }
+    const i64 s = s0 + s1;
+    acc[c] = (int32_t)(s & 0x0FFFFFFF);
+    carry = s >> 28;
+  }
+  acc[NC] += (int32_t)carry;
+}
+
 // the integer held by acc (NC limbs of 28 bits + signed top) as NW two's complement 32-bit limbs
 template <int NA, int NW>
 __device__ __forceinline__ void pack28(const int32_t (&acc)[NA], u32 (&W)[NW]) {
@@ -498,15 +520,55 @@ __device__ __forceinline__ void pack28(const int32_t (&acc)[NA], u32 (&W)[NW]) {
   for (int w = W0 + 2; w < NW; ++w) W[w] = ext;
 }
 
-// out = norm(sum_j A[j] * X[j]) over K = S terms: A pre-aligned and stored as 28-bit magnitude limbs (NL limb
-// planes a28, plane stride aps) with sign masks asg (exponent ea), X entries aligned here (over j).  Shared
-// by both pole-step products.  The exact sum is accumulated in 28-bit limbs.
+// First product column of the truncated products (small_dot, mul_norm): the dropped columns stay at
+// least 60 bits below the rounding position of a sum whose top bit is within 60 bits of the largest
+// possible one.
+template <int L>
+struct ProdCut {
+  static constexpr int NB = 32 * L - 2;
+  static constexpr int NL = R28<L>::NL;
+  static constexpr int C0 = (NB - 132) / 28 > 0 ? (NB - 132) / 28 : 0;
+  static constexpr int NH = 2 * NL - C0;                      // 28-bit columns of U (top one signed)
+  static constexpr int NWH = (28 * (NH - 1) + 32) / 32 + 2;   // two's complement 32-bit limbs holding U
+};
+
+// U = the columns >= C0 of acc (units of 2^(28 C0)) as two's complement limbs; returns whether norm(U) is
+// certainly norm(W) for every W = U 2^(28 C0) + D with |D| < 2^(28 C0 + 38): bits [39, 71) of |U| hold both
+// a 0 and a 1 and bitlen(|U|) >= NB + 72, so adding D changes neither the bit length of |W| nor its bits
+// from the half-unit position of the rounding up.
+template <int L>
+__device__ __forceinline__ bool upper_sure(const int32_t (&acc)[2 * R28<L>::NL], u32 (&U)[ProdCut<L>::NWH]) {
+  constexpr int C0 = ProdCut<L>::C0, NH = ProdCut<L>::NH, NWH = ProdCut<L>::NWH;
+  int32_t hi[NH];
+#pragma unroll
+  for (int c = 0; c < NH; ++c) hi[c] = acc[C0 + c];
+  pack28(hi, U);
+  if constexpr (C0 == 0) {
+    return true;
+  } else {
+    u32 A[NWH];
+#pragma unroll
+    for (int k = 0; k < NWH; ++k) A[k] = U[k];
+    to_mag(A);
+    const u32 w = b3d_alignbit(A[2], A[1], 7u);
+    return bitlen(A) >= ProdCut<L>::NB + 72 && w != 0u && w != 0xFFFFFFFFu;
+  }
+}
+
+// out = norm(sum_j A[j] * X[j]) over K = S <= 16 terms: A pre-aligned and stored as 28-bit magnitude limbs (NL
+// limb planes a28, plane stride aps) with sign masks asg (exponent ea), X entries aligned here (over j).
+// Shared by both pole-step products.  The exact sum W is accumulated in 28-bit limbs, first without the
+// partial products below column C0 (about 40 % of them at L = 34): the upper part U = W - D, where
+// |D| < S (C0 + 1) 2^(28 C0 + 28) <= 2^(28 C0 + 38) for the dropped D (S <= 16, C0 < 64).  When upper_sure
+// holds, norm(U) = norm(W) exactly; otherwise (a sum that cancels far below its terms, or a rare bit
+// pattern) the lane recomputes W in full: the result is always the exactly rounded sum.
 template <int L>
 __device__ __forceinline__ i64 small_dot(const int32_t* __restrict__ a28, const uint32_t* __restrict__ asg, i64 aps,
                                          i64 abase, i64 ea, const int32_t* __restrict__ xm,
                                          const i64* __restrict__ xe, i64 xps, i64 xbase, i64 xstride, int S,
                                          u32 (&out)[L]) {
   constexpr int NL = R28<L>::NL;
+  constexpr int C0 = ProdCut<L>::C0, NWH = ProdCut<L>::NWH;
   i64 E = B3D_EZ;
   for (int j = 0; j < S; ++j) {
     const i64 idx = xbase + j * xstride;
@@ -519,29 +581,48 @@ __device__ __forceinline__ i64 small_dot(const int32_t* __restrict__ a28, const 
     mp::zero(out);
     return B3D_EZ;
   }
+  // one term's operands: the constant's 28-bit limbs and the aligned X entry in 28-bit limbs with the sign
+  // of the product; false if the term is zero
+  auto term = [&](int j, int32_t (&a)[NL], int32_t (&y28)[NL]) -> bool {
+    const i64 idx = xbase + j * xstride;
+    u32 x[L];
+    mp::load(x, xm, xps, idx);
+    if (x[L - 1] == 0u) return false;
+    const u32 sx = mp::to_mag(x);
+    const i64 sh = E - xe[idx];
+    if (sh >= 32 * L) return false;
+    mp::shr(x, (int)sh);
+    u32 au[NL];
+    mp::load(au, a28, aps, abase + j);
+#pragma unroll
+    for (int k = 0; k < NL; ++k) a[k] = (int32_t)au[k];
+    to28(x, sx ^ asg[abase + j], y28);
+    return true;
+  };
   int32_t acc[2 * NL];
 #pragma unroll
   for (int c = 0; c < 2 * NL; ++c) acc[c] = 0;
   for (int j = 0; j < S; ++j) {
-    const i64 idx = xbase + j * xstride;
-    u32 x[L];
-    mp::load(x, xm, xps, idx);
-    if (x[L - 1] == 0u) continue;
-    const u32 sx = mp::to_mag(x);
-    const i64 sh = E - xe[idx];
-    if (sh >= 32 * L) continue;
-    mp::shr(x, (int)sh);
-    u32 au[NL];
-    mp::load(au, a28, aps, abase + j);
     int32_t a[NL], y28[NL];
-#pragma unroll
-    for (int k = 0; k < NL; ++k) a[k] = (int32_t)au[k];
-    to28(x, sx ^ asg[abase + j], y28);
-    mac28(acc, a, y28);
+    if (term(j, a, y28)) mac28_from<C0>(acc, a, y28);
   }
-  u32 W[2 * L + 2];
-  pack28(acc, W);
-  return mp::norm<L, 2 * L + 2>(W, ea + E, out);
+  u32 U[NWH];
+  const bool sure = upper_sure<L>(acc, U);
+  if (B3D_ANY(!sure)) {
+    if (!sure) {
+      int32_t full[2 * NL];
+#pragma unroll
+      for (int c = 0; c < 2 * NL; ++c) full[c] = 0;
+      for (int j = 0; j < S; ++j) {
+        int32_t a[NL], y28[NL];
+        if (term(j, a, y28)) mac28(full, a, y28);
+      }
+      u32 W[2 * L + 2];
+      pack28(full, W);
+      return mp::norm<L, 2 * L + 2>(W, ea + E, out);
+    }
+  }
+  return mp::norm<L, NWH>(U, ea + E + 28 * C0, out);
 }
 
 // a two's complement value of L limbs as 28-bit magnitude limbs and a sign mask (the pole step's constant form)
@@ -557,19 +638,33 @@ __device__ __forceinline__ u32 split28(u32 (&a)[L], u32 (&a28)[R28<L>::NL]) {
 }
 
 // out = norm(a * b) for two's complement a, b of L limbs (both destroyed), exponent e of the product
+// (the columns below ProdCut<L>::C0 first left out, as in small_dot: the dropped part of one product is below
+// (C0 + 1) 2^(28 C0 + 28); the product of two normalised values has no cancellation, so upper_sure fails
+// only on its rare bit pattern, and then the lane completes the product)
 template <int L>
 __device__ __forceinline__ i64 mul_norm(u32 (&a)[L], u32 (&b)[L], i64 e, u32 (&out)[L]) {
   constexpr int NL = R28<L>::NL;
+  constexpr int C0 = ProdCut<L>::C0, NWH = ProdCut<L>::NWH;
   const u32 sa = to_mag(a), sb = to_mag(b);
   int32_t a28[NL], b28[NL], acc[2 * NL];
 #pragma unroll
   for (int c = 0; c < 2 * NL; ++c) acc[c] = 0;
   to28(a, 0u, a28);
   to28(b, sa ^ sb, b28);
-  mac28(acc, a28, b28);
-  u32 W[2 * L + 2];
-  pack28(acc, W);
-  return norm<L, 2 * L + 2>(W, e, out);
+  mac28_from<C0>(acc, a28, b28);
+  u32 U[NWH];
+  const bool sure = upper_sure<L>(acc, U);
+  if (B3D_ANY(!sure)) {
+    if (!sure) {
+#pragma unroll
+      for (int c = 0; c < 2 * NL; ++c) acc[c] = 0;
+      mac28(acc, a28, b28);
+      u32 W[2 * L + 2];
+      pack28(acc, W);
+      return norm<L, 2 * L + 2>(W, e, out);
+    }
+  }
+  return norm<L, NWH>(U, e + 28 * C0, out);
 }
 
 // One Cauchy term on the grid 2^G: acc += sign(d) floor(m 2^32 / |d| / 2^sh), sh = G + 31 - e >= 0, for the

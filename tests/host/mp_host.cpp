@@ -12,6 +12,7 @@
   switch (L) {                                                                           \
     case 2: { constexpr int LL = 2; return CALL; }                                       \
     case 4: { constexpr int LL = 4; return CALL; }                                       \
+    case 6: { constexpr int LL = 6; return CALL; }                                       \
     case 10: { constexpr int LL = 10; return CALL; }                                     \
     case 34: { constexpr int LL = 34; return CALL; }                                     \
     case 50: { constexpr int LL = 50; return CALL; }                                     \

@@ -182,6 +182,43 @@ def test_small_dot(lib, L):
         assert got(r, e) == R.gemm_aligned(am_, ea, xm, xe, L), trial
 
 
+@pytest.mark.parametrize("L", [4, 6, 10, 34, 50])
+def test_small_dot_cancellation(lib, L):
+    """sums far below their terms (near and exact cancellation), sums whose top bit is tens of bits below the
+    largest term product, power-of-two patterns: the truncated product's certainty test must send these
+    to the full product, so the result stays the exactly rounded sum"""
+    rng = random.Random(550 + L)
+    nb = R.NB(L)
+    for trial in range(300):
+        S = rng.randint(2, 6)
+        kind = trial % 4
+        arow = [rp(rng, L, spread=3) for _ in range(S)]
+        xs = [rp(rng, L, spread=3) for _ in range(S)]
+        if kind == 0:                                     # x_1 = -x_0 a_0 / a_1 up to a few low bits
+            arow[1] = (arow[0][0], arow[0][1])
+            xs[1] = (-xs[0][0] + rng.randint(-(1 << rng.randint(0, nb)), 1 << rng.randint(0, nb)), xs[0][1])
+            if xs[1][0] == 0 or abs(xs[1][0]).bit_length() != nb:
+                xs[1] = (-xs[0][0], xs[0][1])
+        elif kind == 1:                                   # the largest a meets a small x and vice versa
+            d = rng.randint(20, 200)
+            arow = [((1 << (nb - 1)) + rng.getrandbits(nb - 2), -nb), ((1 << (nb - 1)) + rng.getrandbits(nb - 2), -nb - d)]
+            xs = [((1 << (nb - 1)) + rng.getrandbits(nb - 2), -nb - d), ((1 << (nb - 1)) + rng.getrandbits(nb - 2), -nb)]
+            S = 2
+        elif kind == 2:                                   # powers of two and all-ones mantissas
+            arow = [(rng.choice([1 << (nb - 1), (1 << nb) - 1, -(1 << (nb - 1))]), rng.randint(-2, 2) - nb)
+                    for _ in range(S)]
+            xs = [(rng.choice([1 << (nb - 1), (1 << (nb - 1)) + 1, (1 << nb) - 1]), rng.randint(-2, 2) - nb)
+                  for _ in range(S)]
+        am_, ea = R.align(arow)
+        A = np.stack([limbs(v, L) for v in am_], 1).view(np.int32).copy()
+        X = np.stack([limbs(v[0], L) for v in xs], 1).view(np.int32).copy()
+        XE = np.array([v[1] for v in xs], np.int64)
+        r = np.zeros(L, np.uint32)
+        e = lib.hc_dot(L, ptr(A), I64(S), I64(0), I64(ea), ptr(X), ptr(XE), I64(S), I64(0), I64(1), S, ptr(r))
+        xm, xe = R.align(xs)
+        assert got(r, e) == R.gemm_aligned(am_, ea, xm, xe, L), (trial, kind)
+
+
 @pytest.mark.parametrize("L", [4, 10, 34])
 def test_mom_window(lib, L):
     rng = random.Random(600 + L)
