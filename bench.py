@@ -40,6 +40,7 @@ def parse():
                     help="tasks of the heaviest group run by the plain per-task path (0: skip)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--serve", type=float, default=30.0, help="seconds of the mixed serving stream (0: skip)")
+    ap.add_argument("--no-json", action="store_true", help="skip the JSON emission sample")
     ap.add_argument("--json-out", default=None)
     return ap.parse_args()
 
@@ -75,13 +76,13 @@ class GroupRun:
         self.plan = Plan(tj12, tj43, TJMAX, ORDER, ((0, 0), (1, 1)), reuse=True)
         self.rec = Recursion(self.plan, None, L, seed=seed)
         self.subs = []
-        Sm, Jh = self.plan.Smax, self.plan.Jh
+        S2, Jh = self.plan.S2, self.plan.Jh
         der = Derivs(self.plan, LAMBDA, L, kept_pole_order=KEPT)     # R = Lambda covers every lambda order
         for sign in (1, -1):
             p0 = (1 - sign) // 2
             out = output_map(p0, L)
             ns = list(range(p0, LAMBDA + 1, 2))
-            hs = [mpf.random_device((2, ORDER + 1, Jh, Sm * Sm, NQ_PER_SIGN), L, seed=seed * 100 + 10 * p0 + i)
+            hs = [mpf.random_device((2, ORDER + 1, Jh, S2, NQ_PER_SIGN), L, seed=seed * 100 + 10 * p0 + i)
                   for i, n in enumerate(ns)]
             self.subs.append((sign, p0, der, out, ns, hs))
         torch.cuda.synchronize()
@@ -159,7 +160,7 @@ def serving_stream(seconds, seed=7):
         ns = list(range(p0, key[4] + 1, 2))
         hk = (key[:8], sign, k % 2)
         if hk not in hcache:
-            hcache[hk] = [mpf.random_device((2, key[3] + 1, plan.Jh, plan.Smax ** 2, 1), key[6], seed=k * 31 + n)
+            hcache[hk] = [mpf.random_device((2, key[3] + 1, plan.Jh, plan.S2, 1), key[6], seed=k * 31 + n)
                           for n in ns]
         return Request(key, sign, hcache[hk], coords=("xt",), d12="0")
 
@@ -254,16 +255,16 @@ def cpu_baseline(runs, seconds=12.0):
     t4 = C.time_unit(0, L, 4, th, part) * 4
     b = (t4 - t1) / 3
     a = t1 - b
-    # pole step: two K-term dots per entry of a task's S x S result, K = the group's S
+    # pole step per entry of a task's SL x SR result: an SR-term dot (R) and an SL-term dot (q L)
     tdot = {k: C.time_unit(1, L, k, th, part) for k in range(1, 6)}
     tot = 0.0
     for g in runs:
         pl = g.plan
-        S2 = pl.Smax ** 2
+        S2 = pl.S2
         rhs = 2 * NQ_PER_SIGN * len(g.subs[0][4])
         entries = sum(lv.nH for lv in pl.levels if lv) * S2
         terms = pl.cauchy_terms() * S2
-        pole = 2 * sum(lv.T for lv in pl.levels if lv) * S2 * tdot[max(1, min(5, pl.Smax))]
+        pole = sum(lv.T for lv in pl.levels if lv) * S2 * (tdot[min(5, pl.SRm)] + tdot[min(5, pl.SLm)])
         tot += rhs * (entries * a + terms * b + pole)
     blocks = sum(g.blocks() for g in runs)
     return {"value": blocks / tot, "unit": "blocks/s", "cores": th, "kind": "port",
@@ -271,6 +272,63 @@ def cpu_baseline(runs, seconds=12.0):
                       f"{b * 1e9:.0f} ns per term, 5-term pole dot {tdot[5] * 1e9:.0f} ns (aggregate per unit), "
                       f"scaled by the workload's recursion entry/term counts; derivative and output stages "
                       f"not counted"}
+
+
+def json_emission(runs, dt_step, blocks_step):
+    """Text emission of the spin files by the library's host formatter (b3d_format_polys), timed on a sample:
+    the spin files of the heaviest group's first task of sign +1 (every kept coefficient of every
+    structure, output and spin), on up to 16 host threads.  The whole step's emission is estimated from the sample's rate and the exact
+    count of coefficients the step's spin files hold (before the truncation of negligible top
+    coefficients, so an upper bound)."""
+    import torch
+    from blocksvibe_amd import b3djson as BJ
+    from blocksvibe_amd import structure as S
+    threads = max(1, min(16, os.cpu_count() or 1))
+    p = BJ.digits10(PREC)
+    eps = BJ.trim_exponent(p)
+    pp = ((0, 0), (1, 1))
+
+    def layout(g, sign_index, qs):
+        """indices of the polynomials that the spin files of tasks qs hold, in the result's
+        (Jo, 2, S2, nq, outputs) order"""
+        sign, p0, der, out, ns, hs = g.subs[sign_index]
+        SR, S2 = g.plan.SRm, g.plan.S2
+        no, nq = len(out.outputs), NQ_PER_SIGN
+        sel = []
+        for jo, tj in enumerate(der.tjs):
+            for k in range(2):
+                nl = len(S.so3(pp[k][0], g.tj12, tj))
+                nr = len(S.so3(pp[k][1], g.tj43, tj))
+                for a in range(nl):
+                    for b in range(nr):
+                        base = ((jo * 2 + k) * S2 + a * SR + b) * nq
+                        sel += [(base + q) * no + o for q in qs for o in range(no)]
+        return sel
+
+    heavy = max(runs, key=lambda g: g.plan.S2)
+    r = heavy.run(keep=True)[0]                                 # (Jo, 2, S2, nq, outputs, Dmax), sign +1
+    L, D = r.m.shape[0], r.e.shape[-1]
+    sel = torch.tensor(layout(heavy, 0, [0]), dtype=torch.int64, device=r.e.device)   # the first task's files
+    m = r.m.reshape(L, -1, D).index_select(1, sel).cpu().numpy()
+    e = r.e.reshape(-1, D).index_select(0, sel).cpu().numpy()
+    t = time.time()
+    bodies = BJ.poly_bodies(m, e, p, eps, threads=threads)
+    dt = time.time() - t
+    nbytes = sum(len(x) for x in bodies)
+    numbers = sum(x.count('",') + 1 for x in bodies)
+    rate = numbers / dt
+    total = 0
+    for g in runs:
+        for si, (sign, p0, der, out, ns, hs) in enumerate(g.subs):
+            total += len(layout(g, si, range(NQ_PER_SIGN))) * der.for_order(LAMBDA - p0).Dout
+    est = total / rate
+    return {"sample": f"group {heavy.tj12},{heavy.tj43}, sign +1, first task's {TJMAX // 2 + 1} spin files: "
+                      f"{len(sel)} polynomials x {D} coefficients, {numbers} kept numbers, "
+                      f"{nbytes / 1e6:.1f} MB of list bodies",
+            "threads": threads, "seconds": dt, "numbers_per_s": rate, "mb_per_s": nbytes / dt / 1e6,
+            "step_numbers_upper_bound": total, "step_emission_s_estimate": est,
+            "blocks_per_s_with_json_overlapped": blocks_step / max(dt_step, est),
+            "note": "emission on host threads while the GPU runs the next groups; file writes not counted"}
 
 
 def main():
@@ -320,7 +378,7 @@ def main():
     res["roofline"] = roofline(timer, res["config"]["workload"])
     print(f"# roofline: {res['roofline']}", file=sys.stderr, flush=True)
     if args.plain_tasks > 0:
-        heavy = max(runs, key=lambda g: g.plan.Smax)
+        heavy = max(runs, key=lambda g: g.plan.S2)
         torch.cuda.synchronize()
         tp = time.time()
         n = heavy.run_plain(task_limit=args.plain_tasks)
@@ -332,6 +390,9 @@ def main():
                              "batched_blocks_per_s_same_group": batched, "batched_over_plain": batched / (nb / tp),
                              "note": "plain: one structure at a time (B = 1 per recursion), same kernels"}
         print(f"# plain path: {res['plain_path']}", file=sys.stderr, flush=True)
+    if not args.no_json:
+        res["json_emission"] = json_emission(runs, dt / args.steps, blocks // args.steps)
+        print(f"# json emission: {res['json_emission']}", file=sys.stderr, flush=True)
     if args.serve > 0:
         res["serving"] = serving_stream(args.serve)
         print(f"# serving: {res['serving']}", file=sys.stderr, flush=True)

@@ -319,7 +319,7 @@ def max_rhs(plan, der, L, free=None):
     import torch
     if free is None:
         free, _ = torch.cuda.mem_get_info()
-    S2, nb = plan.Smax ** 2, 4 * L + 8
+    S2, nb = plan.S2, 4 * L + 8
     per = S2 * nb * ((plan.nphys + 1) + 2 * der.Jo * der.NP * (der.R + 1) + 8 * 2 * der.Jo * der.NP
                      + 2 * (plan.order + 1) * plan.Jh)
     n = max(1, int(0.6 * free / (2 * per)))
@@ -380,7 +380,8 @@ def run_group(tasks, timers):
 
 def gpu_group(tasks, q4s, sfile, mfile, timers):
     """the GPU chain for one group -> {(q4, sign): (spins, pp, {(coord, d12): (m, e) host arrays (L, Jo, 2, S2,
-    outputs, Dmax) / (Jo, 2, S2, outputs, Dmax)}, {(coord, d12): outputs list}, Smax)}"""
+    outputs, Dmax) / (Jo, 2, S2, outputs, Dmax)}, {(coord, d12): outputs list}, SR)} with S2 = SL SR structure
+    entries, entry (a, b) at a SR + b"""
     import torch
     from . import mpf
     from . import setup as SU
@@ -416,14 +417,14 @@ def gpu_group(tasks, q4s, sfile, mfile, timers):
         torch.cuda.synchronize()
         t2 = time.time()
         timers.append((".main.gpu.constants", t2 - t1, 1))
-        Sm, Jh = plan.Smax, plan.Jh
+        SL, SR, Jh = plan.SLm, plan.SRm, plan.Jh
         bmax = max_rhs(plan, der, L)
         chunks = [sts[i:i + bmax] for i in range(0, len(sts), bmax)]
         for ch in chunks:
             nums = []
             for n in ns:
                 dn = der.for_order(T.Lambda - n)
-                hs = [SU.h_inf_tensor(st, [n], T.order, Jh, Sm, L) for st in ch]
+                hs = [SU.h_inf_tensor(st, [n], T.order, Jh, SL, SR, L) for st in ch]
                 h = mpf.cat(hs, 4) if len(hs) > 1 else hs[0]
                 dn.start(len(ch))
                 rec.run(h, hook=dn.accumulate)
@@ -434,7 +435,7 @@ def gpu_group(tasks, q4s, sfile, mfile, timers):
                 m = r.m.cpu().numpy()
                 e = r.e.cpu().numpy()
                 for q, st in enumerate(ch):
-                    ent = results.setdefault((st.q4, st.sign), (tjs, pp, {}, {}, Sm))
+                    ent = results.setdefault((st.q4, st.sign), (tjs, pp, {}, {}, SR))
                     ent[2][(c, d)] = (m[:, :, :, :, q], e[:, :, :, q])
                     ent[3][(c, d)] = o.outputs
         torch.cuda.synchronize()
@@ -461,7 +462,7 @@ def _write_one(tasks, results, job, threads):
     from . import structure as S
     i, tj = job
     t = tasks[i]
-    tjs, pp, vals, outs, Sm = results[(tuple(t.q4), t.sign)]
+    tjs, pp, vals, outs, SR = results[(tuple(t.q4), t.sign)]
     jo = tjs.index(tj)
     p = BJ.digits10(t.precision)
     structs = [(len(S.so3(pp[k][0], t.two_j12, tj)), len(S.so3(pp[k][1], t.two_j43, tj))) for k in range(2)]
@@ -477,7 +478,7 @@ def _write_one(tasks, results, job, threads):
                 for b in range(structs[k][1]):
                     for o in range(no):
                         where[(k, a, b, o)] = len(sel)
-                        sel.append((k * S2 + a * Sm + b) * no + o)
+                        sel.append((k * S2 + a * SR + b) * no + o)
         bodies = BJ.poly_bodies(mj, ej, p, BJ.trim_exponent(p), select=np.array(sel, dtype=np.int64),
                                 threads=threads)
         rows_k = []

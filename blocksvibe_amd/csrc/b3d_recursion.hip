@@ -6,11 +6,12 @@
 //              them per column, divides by the one-limb integer D' - D_i, sums exactly, rounds once and
 //              adds the h_inf entry (k_cauchy_rows when the rows, padded to whole waves, keep at least
 //              3/4 of the lanes busy: the group's data then sits in scalar registers; k_cauchy otherwise);
-//   k_pole_r   Y[t, b0, a, k] = sum_b Ralign[R_t][b0][b] * H[hrow_t][a, b][k]
-//   k_pole_l   res[slot_t][a0, b0][k] = sum_a QLalign[qL_t][a0][a] * Y[t, b0, a, k]
+//   k_pole_r   Y[t, b0, a, k] = sum_{b < SR} Ralign[R_t][b0][b] * H[hrow_t][a, b][k]
+//   k_pole_l   res[slot_t][a0, b0][k] = sum_{a < SL} QLalign[qL_t][a0][a] * Y[t, b0, a, k]
 //
-// Layouts (limb planes, entries innermost = right-hand side k):
-//   slot buffer res: (L, nslots, S*S, B), H: (L, nH, S*S, B), Y: (L, T, S, S, B),
+// Layouts (limb planes, entries innermost = right-hand side k), SL x SR structure entries per residue (left
+// a < SL, right b < SR), S >= SL, SR the row stride of the constant tables:
+//   slot buffer res: (L, nslots, SL*SR, B), H: (L, nH, SL*SR, B), Y: (L, T, SR, SL, B),
 //   R / qL constants: aligned per row, as 28-bit magnitude limb planes (NL, ntab, S, S) + sign masks
 //   (ntab, S, S) (b3d_mpf_to28), (ntab, S) row exponents.
 #include "common.hpp"
@@ -73,20 +74,20 @@ __global__ __launch_bounds__(256) void k_cauchy(const int32_t* __restrict__ rm, 
 template <int L>
 __global__ __launch_bounds__(256) void k_pole_r(const int32_t* __restrict__ Hm, const i64* __restrict__ He, i64 hps,
                                                 const int64_t* __restrict__ thrw, const int32_t* __restrict__ tR,
-                                                i64 T, int S, i64 B, const int32_t* __restrict__ R28,
+                                                i64 T, int SL, int SR, int S, i64 B, const int32_t* __restrict__ R28,
                                                 const uint32_t* __restrict__ Rsg, const i64* __restrict__ Re,
                                                 i64 rps, int32_t* __restrict__ Ym, i64* __restrict__ Ye, i64 yps) {
-  const i64 SB = (i64)S * B;
+  const i64 LB = (i64)SL * B, NE = (i64)SR * LB;   // entries of one task's Y (b0, a, k)
   const unsigned lb = b3d_xcd_block(blockIdx.x, gridDim.x);   // a task's outputs stay on one XCD's L2
-  for (i64 t = lb * (i64)blockDim.x + threadIdx.x; t < T * S * SB; t += (i64)gridDim.x * blockDim.x) {
+  for (i64 t = lb * (i64)blockDim.x + threadIdx.x; t < T * NE; t += (i64)gridDim.x * blockDim.x) {
     const i64 k = t % B;
-    const i64 a = (t / B) % S;
-    const i64 b0 = (t / SB) % S;
-    const i64 task = t / (S * SB);
+    const i64 a = (t / B) % SL;
+    const i64 b0 = (t / LB) % SR;
+    const i64 task = t / NE;
     const i64 r = tR[task];
-    const i64 xbase = thrw[task] * S * SB + a * SB + k;  // H[hrow][a*S + b][k], b stride B
+    const i64 xbase = thrw[task] * NE + a * SR * B + k;  // H[hrow][a*SR + b][k], b stride B
     u32 out[L];
-    const i64 e = mp::small_dot<L>(R28, Rsg, rps, (r * S + b0) * S, Re[r * S + b0], Hm, He, hps, xbase, B, S, out);
+    const i64 e = mp::small_dot<L>(R28, Rsg, rps, (r * S + b0) * S, Re[r * S + b0], Hm, He, hps, xbase, B, SR, out);
     mp::store(out, Ym, yps, t);  // (task, b0, a, k)
     Ye[t] = e;
   }
@@ -95,21 +96,21 @@ __global__ __launch_bounds__(256) void k_pole_r(const int32_t* __restrict__ Hm, 
 template <int L>
 __global__ __launch_bounds__(256) void k_pole_l(const int32_t* __restrict__ Ym, const i64* __restrict__ Ye, i64 yps,
                                                 const int32_t* __restrict__ tQL, const int64_t* __restrict__ tslot,
-                                                i64 T, int S, i64 B, const int32_t* __restrict__ Q28,
+                                                i64 T, int SL, int SR, int S, i64 B, const int32_t* __restrict__ Q28,
                                                 const uint32_t* __restrict__ Qsg, const i64* __restrict__ Qe, i64 qps,
                                                 int32_t* __restrict__ rm, i64* __restrict__ re, i64 rps) {
-  const i64 SB = (i64)S * B;
+  const i64 RB = (i64)SR * B, NE = (i64)SL * RB;   // entries of one task's result (a0, b0, k)
   const unsigned lb = b3d_xcd_block(blockIdx.x, gridDim.x);   // a task's outputs stay on one XCD's L2
-  for (i64 t = lb * (i64)blockDim.x + threadIdx.x; t < T * S * SB; t += (i64)gridDim.x * blockDim.x) {
+  for (i64 t = lb * (i64)blockDim.x + threadIdx.x; t < T * NE; t += (i64)gridDim.x * blockDim.x) {
     const i64 k = t % B;
-    const i64 b0 = (t / B) % S;
-    const i64 a0 = (t / SB) % S;
-    const i64 task = t / (S * SB);
+    const i64 b0 = (t / B) % SR;
+    const i64 a0 = (t / RB) % SL;
+    const i64 task = t / NE;
     const i64 q = tQL[task];
-    const i64 xbase = (task * S + b0) * SB + k;  // Y[task][b0][a][k], a stride B
+    const i64 xbase = (task * SR + b0) * SL * B + k;  // Y[task][b0][a][k], a stride B
     u32 out[L];
-    const i64 e = mp::small_dot<L>(Q28, Qsg, qps, (q * S + a0) * S, Qe[q * S + a0], Ym, Ye, yps, xbase, B, S, out);
-    const i64 o = tslot[task] * S * SB + (a0 * S + b0) * B + k;
+    const i64 e = mp::small_dot<L>(Q28, Qsg, qps, (q * S + a0) * S, Qe[q * S + a0], Ym, Ye, yps, xbase, B, SL, out);
+    const i64 o = tslot[task] * NE + (a0 * SR + b0) * B + k;
     mp::store(out, rm, rps, o);
     re[o] = e;
   }
@@ -134,28 +135,28 @@ int b3d_cauchy_level(int L, const int32_t* rm, const int64_t* re, int64_t rps, i
   return b3d::check_launch("b3d_cauchy_level");
 }
 
-int b3d_pole_level(int L, int S, int64_t B, int64_t T, const int32_t* Hm, const int64_t* He, int64_t hps,
-                   const int64_t* thrw, const int32_t* tR, const int32_t* tQL, const int64_t* tslot,
+int b3d_pole_level(int L, int SL, int SR, int S, int64_t B, int64_t T, const int32_t* Hm, const int64_t* He,
+                   int64_t hps, const int64_t* thrw, const int32_t* tR, const int32_t* tQL, const int64_t* tslot,
                    const int32_t* R28, const uint32_t* Rsg, const int64_t* Re, int64_t rtps, const int32_t* Q28,
                    const uint32_t* Qsg, const int64_t* Qe, int64_t qps, const int8_t* Rd8, const int32_t* Rcorr,
                    const int8_t* Qd8, const int32_t* Qcorr, int32_t* Ym, int64_t* Ye, int64_t yps, int32_t* rm,
                    int64_t* re, int64_t rps, void* stream) {
   if (T <= 0 || B <= 0) return 0;
-  if (S < 1 || S > 16) {
-    b3d::set_error("b3d_pole_level: structure count S must be 1..16");
+  if (S < 1 || S > 16 || SL < 1 || SR < 1 || SL > S || SR > S) {
+    b3d::set_error("b3d_pole_level: need 1 <= SL, SR <= S <= 16");
     return 2;
   }
   if (b3d::check_planes("b3d_pole_level", {hps, rtps, qps, yps, rps})) return 2;
   hipStream_t s = (hipStream_t)stream;
-  if (Rd8 && Rcorr && Qd8 && Qcorr && (int64_t)S * B >= 32 && S <= b3d::pole_mma_max_s(L)) {
-    // at least 32 rows share each constant row: int8 MFMA products
+  if (Rd8 && Rcorr && Qd8 && Qcorr && SL == S && SR == S && (int64_t)S * B >= 32 && S <= b3d::pole_mma_max_s(L)) {
+    // square structure blocks, at least 32 rows sharing each constant row: int8 MFMA products
     if (b3d::pole_mma(L, Hm, He, hps, thrw, tR, Rd8, Rcorr, Re, nullptr, T, S, B, Ym, Ye, yps, s)) return 1;
     return b3d::pole_mma(L, Ym, Ye, yps, nullptr, tQL, Qd8, Qcorr, Qe, tslot, T, S, B, rm, re, rps, s);
   }
-  const int64_t n = T * S * S * B;
-  B3D_DISPATCH_L(L, k_pole_r<LL><<<dim3(grid_for(n, 256)), dim3(256), 0, s>>>(Hm, He, hps, thrw, tR, T, S, B, R28, Rsg, Re, rtps, Ym, Ye, yps));
+  const int64_t n = T * SL * SR * B;
+  B3D_DISPATCH_L(L, k_pole_r<LL><<<dim3(grid_for(n, 256)), dim3(256), 0, s>>>(Hm, He, hps, thrw, tR, T, SL, SR, S, B, R28, Rsg, Re, rtps, Ym, Ye, yps));
   if (b3d::check_launch("b3d_pole_level (R)")) return 1;
-  B3D_DISPATCH_L(L, k_pole_l<LL><<<dim3(grid_for(n, 256)), dim3(256), 0, s>>>(Ym, Ye, yps, tQL, tslot, T, S, B, Q28, Qsg, Qe, qps, rm, re, rps));
+  B3D_DISPATCH_L(L, k_pole_l<LL><<<dim3(grid_for(n, 256)), dim3(256), 0, s>>>(Ym, Ye, yps, tQL, tslot, T, SL, SR, S, B, Q28, Qsg, Qe, qps, rm, re, rps));
   return b3d::check_launch("b3d_pole_level (L)");
 }
 

@@ -230,8 +230,7 @@ class Derivs:
     # --------------------------------------------------------------------------------------- moments
     def start(self, B, window=8):
         """zero the moment accumulators for B right-hand sides"""
-        Sm = self.plan.Smax
-        self.B, self.N = B, Sm * Sm * B
+        self.B, self.N = B, self.plan.S2 * B
         self.nrows = 2 * self.Jo * self.NP
         self.mom = mpf.zeros((self.nrows, self.N, self.R + 1), self.L)
         self.W = window
@@ -266,15 +265,14 @@ class Derivs:
 
     # --------------------------------------------------------------------------------------- final maps
     def finish(self, h_inf, chunk=None):
-        """h_inf: MPF (2, order + 1, Jh, Smax^2, B).  Returns numerators MPF (Jo, Dout, R+1, 2, Smax^2, B):
-        [spin, x^k coefficient, log r derivative e, parity, structure a*Smax+b, right-hand side]"""
+        """h_inf: MPF (2, order + 1, Jh, SL SR, B).  Returns numerators MPF (Jo, Dout, R+1, 2, SL SR, B):
+        [spin, x^k coefficient, log r derivative e, parity, structure a*SR+b, right-hand side]"""
         if self.pending:
             self._flush()
         self.stage = None
         L, R, Jo, NP, order = self.L, self.R, self.Jo, self.NP, self.plan.order
         R1, Rw = R + 1, max(R, 1)
-        Sm = self.plan.Smax
-        S2, B, N = Sm * Sm, self.B, self.N
+        S2, B, N = self.plan.S2, self.B, self.N
         Nc = 2 * N
         jidx, cidx, hidx, gidx, cb = self._fc
         H = mpf.MPF(h_inf.m[:, :, :order + 1][:, :, :, jidx], h_inf.e[:, :order + 1][:, :, jidx])  # (2,O1,Jo,S2,B)

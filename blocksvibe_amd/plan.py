@@ -47,7 +47,12 @@ class Plan:
         self.SL = np.array([[len(S.so3(self.pp[p][0], two_j12, int(tj))) for tj in tj_all] for p in range(2)])
         self.SR = np.array([[len(S.so3(self.pp[p][1], two_j43, int(tj))) for tj in tj_all] for p in range(2)])
         nsp = (two_j_max + 2 * order - b) // 2 + 1                 # spins up to two_j_max + 2 order
-        self.Smax = int(max(self.SL[:, :nsp].max(), self.SR[:, :nsp].max()))
+        # structure entries per residue: SLm left x SRm right (a residue block (a, b) sits at a * SRm + b);
+        # Smax = max of the two is the row stride of the M-matrix tables
+        self.SLm = int(self.SL[:, :nsp].max())
+        self.SRm = int(self.SR[:, :nsp].max())
+        self.S2 = self.SLm * self.SRm
+        self.Smax = max(self.SLm, self.SRm)
         self.Jh = len(range(b, two_j_max + 2 * order + 1, 2))      # spins of h_inf
         # logical slot layout
         self.level_tjs, self.level_npol, self.level_first = [], [], []
@@ -255,10 +260,10 @@ class Plan:
         return int(self.phys[logical])
 
     def counts(self):
-        """(Cauchy output entries, pole-step output entries) per right-hand side (structures padded to
-        Smax^2) -- what one run computes"""
-        h = sum(lev.nH for lev in self.levels if lev) * self.Smax ** 2
-        r = sum(lev.T for lev in self.levels if lev) * self.Smax ** 2
+        """(Cauchy output entries, pole-step output entries) per right-hand side (SLm x SRm structure entries
+        per residue) -- what one run computes"""
+        h = sum(lev.nH for lev in self.levels if lev) * self.S2
+        r = sum(lev.T for lev in self.levels if lev) * self.S2
         return h, r
 
     def cauchy_terms(self):

@@ -3,11 +3,12 @@
 For one operator (a Plan) the level loop runs, per level m, two libb3dhip kernels' worth of work:
 
   Cauchy step  H(src, m, j', D') = h_inf[src][m][j'] + sum_i 2 / (D' - D_i) res[src][m - shift_i][j'][i]
-  pole step    res[p][m][j][P]   = Q(P, j) L_P H R_P^T        (S x S structure matrices)
+  pole step    res[p][m][j][P]   = Q(P, j) L_P H R_P^T        (SL x SL left, SR x SR right structure matrices)
 
 over all right-hand sides (four-point structures x lambda orders) at once: the batch B is the innermost
-dimension of every array.  Constants (M matrices, Q values, their pre-aligned forms) are built once per
-operator and reused by every run().
+dimension of every array.  A residue holds SL x SR structure entries (the operator's left and right
+structure counts), not a padded Smax x Smax square.  Constants (M matrices, Q values, their pre-aligned
+forms) are built once per operator and reused by every run().
 """
 import os
 import time
@@ -154,12 +155,12 @@ class Recursion:
         self.maxT = max((lv["T"] for lv in live), default=1)
 
     def run(self, h_inf, hook=None, prof=None, res=None, timer=None):
-        """h_inf: MPF (2, order + 1, Jh, Smax^2, B).  Returns the residue slot buffer MPF
-        (nphys + 1, Smax^2, B) (physical slots; the last one is the zero slot).  hook(m, res) runs after each
-        level (e.g. the derivative moments).  timer: a list that receives (start event, end event,
-        algorithmic bytes) of every Cauchy launch, events recorded on the launch stream."""
+        """h_inf: MPF (2, order + 1, Jh, SL SR, B) (structure entry (a, b) at a SR + b).  Returns the residue
+        slot buffer MPF (nphys + 1, SL SR, B) (physical slots; the last one is the zero slot).  hook(m, res)
+        runs after each level (e.g. the derivative moments).  timer: a list that receives (start event, end
+        event, algorithmic bytes) of every Cauchy launch, events recorded on the launch stream."""
         plan, L, Sm = self.plan, self.L, self.plan.Smax
-        S2 = Sm * Sm
+        SL, SR, S2 = plan.SLm, plan.SRm, plan.S2
         assert h_inf.shape[:4] == (2, plan.order + 1, plan.Jh, S2), (h_inf.shape, plan.order, plan.Jh, S2)
         B = h_inf.shape[-1]
         N = S2 * B
@@ -170,7 +171,7 @@ class Recursion:
             res.m.zero_()
             res.e.fill_(mpf.EZ)
         H = mpf.zeros((self.maxH, S2, B), L)
-        Y = mpf.zeros((self.maxT, Sm, Sm, B), L)
+        Y = mpf.zeros((self.maxT, SR, SL, B), L)
         st = mpf.stream_ptr()
         P = lambda t: t.data_ptr()
         for lv in self.levels:
@@ -191,7 +192,7 @@ class Recursion:
                 torch.cuda.synchronize()
                 t1 = time.time()
                 prof["cauchy"] = prof.get("cauchy", 0.0) + t1 - t0
-            _lib.call("b3d_pole_level", L, Sm, B, lv["T"], P(H.m), P(H.e), H.numel(), P(lv["t_hrow"]),
+            _lib.call("b3d_pole_level", L, SL, SR, Sm, B, lv["T"], P(H.m), P(H.e), H.numel(), P(lv["t_hrow"]),
                       P(lv["t_R"]), P(lv["t_qL"]), P(lv["t_slot"]), P(self.R28), P(self.Rsg), P(self.Ralign_e),
                       self.R28[0].numel(), P(self.QL28), P(self.QLsg), P(self.QLalign_e), self.QL28[0].numel(),
                       P(self.Rd8) if self.mma else 0, P(self.Rcorr) if self.mma else 0,

@@ -84,12 +84,12 @@ class _Operator:
     def per_rhs_bytes(self, p0):
         d = self.derivs_for(p0)
         pl, L = self.plan, self.key[6]
-        S2, nb = pl.Smax ** 2, 4 * L + 8
+        S2, nb = pl.S2, 4 * L + 8
         return S2 * nb * ((pl.nphys + 1) + (d.R + 1 + 8) * 2 * d.Jo * d.NP)
 
     def h_bytes(self):
         pl, L = self.plan, self.key[6]
-        return pl.Smax ** 2 * (4 * L + 8) * 2 * (pl.order + 1) * pl.Jh
+        return pl.S2 * (4 * L + 8) * 2 * (pl.order + 1) * pl.Jh
 
 
 _MODE_THREAD_LOCAL, _MODE_RELAXED = 1, 2     # hipStreamCaptureMode

@@ -173,20 +173,21 @@ def save_m(path, cache):
     os.replace(tmp, cache)
 
 
-def h_inf_tensor(task, ns, order, Jh, Smax, L):
-    """h_inf MPF (2, order + 1, Jh, Smax^2, len(ns)) for the lambda orders ns of a task"""
+def h_inf_tensor(task, ns, order, Jh, SL, SR, L):
+    """h_inf MPF (2, order + 1, Jh, SL SR, len(ns)) for the lambda orders ns of a task (structure entry (a, b)
+    at a SR + b)"""
     B = len(ns)
-    out = mpf.zeros((2 * (order + 1) * Jh * Smax * Smax * B,), L)
+    out = mpf.zeros((2 * (order + 1) * Jh * SL * SR * B,), L)
     for k, n in enumerate(ns):
         idx, v = task.h[n]
         s, m, j, a, b = idx.T
-        ok = (m <= order) & (j < Jh) & (a < Smax) & (b < Smax)
-        flat = ((((s * (order + 1) + m) * Jh + j) * Smax + a) * Smax + b) * B + k
+        ok = (m <= order) & (j < Jh) & (a < SL) & (b < SR)
+        flat = ((((s * (order + 1) + m) * Jh + j) * SL + a) * SR + b) * B + k
         fi = torch.from_numpy(flat[ok]).to(mpf.dev())
         okt = torch.from_numpy(ok).to(mpf.dev())
         out.m[:, fi] = v.m[:, okt]
         out.e[fi] = v.e[okt]
-    return out.reshape(2, order + 1, Jh, Smax * Smax, B)
+    return out.reshape(2, order + 1, Jh, SL * SR, B)
 
 
 # ------------------------------------------------------------------------------------ synthetic files

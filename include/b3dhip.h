@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-#define B3D_ABI_VERSION 8
+#define B3D_ABI_VERSION 9
 
 /* Last error message of the calling thread (empty if none). */
 const char* b3d_last_error(void);
@@ -73,7 +73,7 @@ int b3d_mpf_align_rows(int L, const int32_t* m, const int64_t* e, int64_t ps, in
 /* ---- radial recursion, one level (replaces the body of reference Recursion.run,
  *      blocksvibe/recursion_gpu.py:446-476: cauchy_kernel cauchy.py:73-87, _pole :483-492) ---- */
 
-/* Cauchy step.  Slot buffer rm/re (plane stride rps, N = S*S*B columns per slot); h_inf hm/he (plane
+/* Cauchy step.  Slot buffer rm/re (plane stride rps, N = SL*SR*B columns per slot); h_inf hm/he (plane
  * stride hps, N columns per row); groups g < nb: ghrow[g] = h_inf row, col[g*Kc + i] physical slots of
  * the prime poles (i < ncol[g]), cold[g*Kc + i] their doubled positions D_i; H rows h < nH: rg[h] group,
  * rd[h] doubled D'.  Output H (nH, N) at om/oe (plane stride ops).  Requires |D' - D_i| < 2^15. */
@@ -82,16 +82,19 @@ int b3d_cauchy_level(int L, const int32_t* rm, const int64_t* re, int64_t rps, i
                      int64_t nH, const int64_t* col, const int32_t* cold, const int32_t* ncol, int Kc, int32_t* om,
                      int64_t* oe, int64_t ops, void* stream);
 
-/* Pole step for T tasks: Y = Ralign[tR[t]] . H[thrw[t]]^T, then res[tslot[t]] = QLalign[tQL[t]] . Y^T,
- * S x S structure matrices, B right-hand sides.  Ralign/QLalign are given in the form b3d_mpf_to28 makes of
- * the row-aligned mantissas: 28-bit magnitude limb planes R28/Q28 (b3d_limbs28(L) planes of (ntab, S, S),
- * plane strides rtps/qps) and sign masks Rsg/Qsg (ntab, S, S), with row exponents Re/Qe (ntab, S); and in
- * the form b3d_pole_tables makes (digits Rd8/Qd8, corrections Rcorr/Qcorr; may be null).  With the digit
- * tables, S*B >= 32 and S small enough for the MFMA path's LDS staging (6 at L = 34) the products run on
- * int8 MFMA, otherwise on the VALU; both give the same bits.
- * Y: scratch (L, T, S, S, B), plane stride yps. */
-int b3d_pole_level(int L, int S, int64_t B, int64_t T, const int32_t* Hm, const int64_t* He, int64_t hps,
-                   const int64_t* thrw, const int32_t* tR, const int32_t* tQL, const int64_t* tslot,
+/* Pole step for T tasks: Y = Ralign[tR[t]] . H[thrw[t]]^T, then res[tslot[t]] = QLalign[tQL[t]] . Y^T, over
+ * residue blocks of SL x SR structure entries (left a < SL, right b < SR; entry (a, b) of a residue at
+ * (a SR + b) B + k) with B right-hand sides: Y[t][b0][a][k] = sum_{b < SR} R[b0][b] H[a][b][k],
+ * res[a0][b0][k] = sum_{a < SL} QL[a0][a] Y[t][b0][a][k].  Ralign/QLalign are (ntab, S, S) tables, S >= SL, SR
+ * (rows and entries beyond SR resp. SL are never read), given in the form b3d_mpf_to28 makes of the row-aligned
+ * mantissas: 28-bit magnitude limb planes R28/Q28 (b3d_limbs28(L) planes, plane strides rtps/qps) and sign
+ * masks Rsg/Qsg (ntab, S, S), with row exponents Re/Qe (ntab, S); and in the form b3d_pole_tables makes
+ * (digits Rd8/Qd8, corrections Rcorr/Qcorr; may be null).  With the digit tables, SL = SR = S, S*B >= 32 and S
+ * small enough for the MFMA path's LDS staging (6 at L = 34) the products run on int8 MFMA, otherwise on the
+ * VALU; both give the same bits.  Y: scratch (L, T, SR, SL, B), plane stride yps.  The reference pads every
+ * residue to Smax x Smax (recursion_gpu.py:483-510); here only the SL x SR entries are stored and computed. */
+int b3d_pole_level(int L, int SL, int SR, int S, int64_t B, int64_t T, const int32_t* Hm, const int64_t* He,
+                   int64_t hps, const int64_t* thrw, const int32_t* tR, const int32_t* tQL, const int64_t* tslot,
                    const int32_t* R28, const uint32_t* Rsg, const int64_t* Re, int64_t rtps, const int32_t* Q28,
                    const uint32_t* Qsg, const int64_t* Qe, int64_t qps, const int8_t* Rd8, const int32_t* Rcorr,
                    const int8_t* Qd8, const int32_t* Qcorr, int32_t* Ym, int64_t* Ye, int64_t yps, int32_t* rm,

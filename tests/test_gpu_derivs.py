@@ -43,8 +43,7 @@ def test_derivs(case):
     rec.run(h, hook=der.accumulate)
     num = der.finish(h, chunk=2)
     got = num.to_host()
-    Sm = plan.Smax
-    S2 = Sm * Sm
+    SR, S2 = plan.SRm, plan.S2
     Dout = der.Dout
     mp.mp.prec = 32 * L + 64
     worst, cnt = 0, 0
@@ -60,7 +59,7 @@ def test_derivs(case):
                         ref = DR.numerators(hf, rf, tj, p, a, b, c["order"], Rr, c["tj12"], c["tj43"], kept=c["kept"])
                         for e, poly in enumerate(ref):
                             big = max(abs(x) for x in poly)
-                            g = [R.value(*got[((((jo * Dout + i) * (Rr + 1) + e) * 2 + p) * S2 + a * Sm + b) * B + k])
+                            g = [R.value(*got[((((jo * Dout + i) * (Rr + 1) + e) * 2 + p) * S2 + a * SR + b) * B + k])
                                  for i in range(Dout)]
                             assert all(x == 0 for x in g[len(poly):]), (jo, p, a, b, e)
                             if big == 0:

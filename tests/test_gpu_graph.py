@@ -13,7 +13,7 @@ def _req(key, sign, plan, L, seed):
     from blocksvibe_amd.serve import Request
     order, Lam = key[3], key[4]
     p0 = (1 - sign) // 2
-    hs = [mpf.random_device((2, order + 1, plan.Jh, plan.Smax ** 2, 1), L, seed=seed + n)
+    hs = [mpf.random_device((2, order + 1, plan.Jh, plan.S2, 1), L, seed=seed + n)
           for n in range(p0, Lam + 1, 2)]
     return lambda: Request(key, sign, hs, coords=("xt",), d12="1.5")
 

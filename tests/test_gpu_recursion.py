@@ -3,8 +3,10 @@ initial data with several right-hand sides (reference tests/test_blocks_recursio
   * bit for bit against oracle.recursion_ref.recursion_bits (the kernels' exact rounding semantics);
   * within 2^-(32L - 64) of the exact rational recursion relative to each (pole, spin) block;
   * with slot recycling (reuse=True) every residue, captured right after its level, is identical.
-Rows whose Smax^2 B columns fill whole waves to at least 3/4 take the row-uniform Cauchy kernel (cases with 63
-and 64 columns), the others the general one; with Smax B >= 32 the pole step runs on int8 MFMA (Smax 5, B 7).
+Residues hold SL x SR structure entries (left and right structure counts; asymmetric cases 3 x 2, 2 x 1,
+4 x 2).  Rows whose SL SR B columns fill whole waves to at least 3/4 take the row-uniform Cauchy kernel (cases
+with 63 and 64 columns), the others the general one; with SL = SR = S and S B >= 32 the pole step runs on int8
+MFMA (S 5, B 7).
 """
 import random
 from fractions import Fraction as F
@@ -19,16 +21,16 @@ from oracle import structure_ref as OS
 pytestmark = pytest.mark.gpu
 
 CASES = [dict(tj12=4, tj43=2, tjmax=6, order=7, L=10, B=3, seed=1, pp=((0, 1), (1, 0))),
-         # Smax 3, B 7: rows of 63 columns on 64 lanes (one idle lane per wave), the row-uniform Cauchy kernel
-         dict(tj12=4, tj43=2, tjmax=6, order=7, L=10, B=7, seed=6, pp=((0, 1), (1, 0))),
-         # Smax 5, B 3: 75 columns per row, waves inside one row and waves across two in the general kernel
+         # 3 x 3 structures, B 7: rows of 63 columns on 64 lanes (one idle lane per wave), row-uniform Cauchy
+         dict(tj12=4, tj43=4, tjmax=6, order=7, L=10, B=7, seed=6, pp=((0, 1), (1, 0))),
+         # 5 x 5, B 3: 75 columns per row, waves inside one row and waves across two in the general kernel
          dict(tj12=8, tj43=8, tjmax=4, order=6, L=10, B=3, seed=2, pp=((0, 1), (1, 0))),
          dict(tj12=3, tj43=1, tjmax=5, order=6, L=6, B=2, seed=3, pp=((0, 1), (1, 0))),
          dict(tj12=2, tj43=2, tjmax=6, order=8, L=34, B=2, seed=4, pp=((0, 0), (1, 1))),
-         # Smax 5, B 7: 35 rows per constant row, the pole step on int8 MFMA
+         # 5 x 5, B 7: 35 rows per constant row, the pole step on int8 MFMA
          dict(tj12=8, tj43=8, tjmax=4, order=6, L=10, B=7, seed=7, pp=((0, 1), (1, 0))),
-         # Smax 4, B 4: 64 columns per row, the row-uniform Cauchy kernel
-         dict(tj12=6, tj43=2, tjmax=4, order=5, L=34, B=4, seed=5, pp=((0, 1), (1, 0)))]
+         # 4 x 2, B 8: 64 columns per row, the row-uniform Cauchy kernel
+         dict(tj12=6, tj43=2, tjmax=4, order=5, L=34, B=8, seed=5, pp=((0, 1), (1, 0)))]
 
 
 def random_operator(plan, tj12, tj43, pp, order, B, seed):
@@ -54,14 +56,15 @@ def random_operator(plan, tj12, tj43, pp, order, B, seed):
 
 
 def h_tensor(H, plan, L):
+    """h_inf (2, order + 1, Jh, SL SR, B), structure entry (a, b) at a SR + b"""
     from blocksvibe_amd import mpf
-    Sm, O1 = plan.Smax, plan.order + 1
+    SL, SR, O1 = plan.SLm, plan.SRm, plan.order + 1
     B = len(H)
-    pairs = [(0, R.EZ)] * (2 * O1 * plan.Jh * Sm * Sm * B)
+    pairs = [(0, R.EZ)] * (2 * O1 * plan.Jh * SL * SR * B)
     for k, h in enumerate(H):
         for (s, m, j, a, b), v in h.items():
-            pairs[((((s * O1 + m) * plan.Jh + j) * Sm + a) * Sm + b) * B + k] = R.from_fraction(v, L)
-    return mpf.from_pairs(pairs, L, (2, O1, plan.Jh, Sm * Sm, B))
+            pairs[((((s * O1 + m) * plan.Jh + j) * SL + a) * SR + b) * B + k] = R.from_fraction(v, L)
+    return mpf.from_pairs(pairs, L, (2, O1, plan.Jh, SL * SR, B))
 
 
 @pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c['tj12']}-{c['tj43']}-{c['tjmax']}-{c['order']}-L{c['L']}")
@@ -73,10 +76,10 @@ def test_recursion_bits_and_exact(case):
     plan = Plan(c["tj12"], c["tj43"], c["tjmax"], c["order"], pp, reuse=False)
     plan_r = Plan(c["tj12"], c["tj43"], c["tjmax"], c["order"], pp, reuse=True)
     M, H = random_operator(plan, c["tj12"], c["tj43"], pp, c["order"], B, c["seed"])
-    print(f"case {c}: {sum(l.T for l in plan.levels if l)} tasks, Smax {plan.Smax}", flush=True)
+    print(f"case {c}: {sum(l.T for l in plan.levels if l)} tasks, {plan.SLm} x {plan.SRm} structures", flush=True)
     h = h_tensor(H, plan, L)
     rec = Recursion(plan, M, L)
-    rec.mma = True                       # the int8 MFMA pole step where Smax B >= 32, the VALU one elsewhere
+    rec.mma = True                       # the int8 MFMA pole step where SL = SR = S, S B >= 32, the VALU one elsewhere
     res = rec.run(h)
     got = res.to_host()
     # slot reuse: capture the slots written at each level right after it
@@ -89,8 +92,7 @@ def test_recursion_bits_and_exact(case):
         cap[m] = (lev.t_slot.copy(), r.take(idx).to_host())
 
     rec_r.run(h, hook=hook)
-    Sm = plan.Smax
-    S2 = Sm * Sm
+    Sm, SR, S2 = plan.Smax, plan.SRm, plan.S2
     Mp = {k: [[R.from_fraction(x, L) for x in row] for row in v] for k, v in M.items()}
     worst = 0
     for k in range(B):
@@ -101,8 +103,11 @@ def test_recursion_bits_and_exact(case):
         for key, v in exact.items():
             blk[key[:4]] = max(blk.get(key[:4], 0), abs(v))
         for (p, m, j, q, a, b), v in bits.items():
+            if a >= plan.SLm or b >= SR:                 # the oracle pads to Smax x Smax: zeros there
+                assert v == (0, R.EZ), ("padding", (p, m, j, q, a, b), v)
+                continue
             slot = rec.slot(p, m, j, q)
-            g = got[(slot * S2 + a * Sm + b) * B + k]
+            g = got[(slot * S2 + a * SR + b) * B + k]
             assert g == v, ("bits", k, (p, m, j, q, a, b), g, v)
             ex = exact.get((p, m, j, q, a, b))
             if ex is not None and blk[(p, m, j, q)]:

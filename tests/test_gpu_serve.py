@@ -22,7 +22,7 @@ def _requests(L):
             p0 = (1 - sign) // 2
             ns = list(range(p0, Lam + 1, 2))
             seed = 1000 * op_id + 10 * i
-            specs.append((key, sign, [mpf.random_device((2, order + 1, plan.Jh, plan.Smax ** 2, 1), L, seed=seed + n)
+            specs.append((key, sign, [mpf.random_device((2, order + 1, plan.Jh, plan.S2, 1), L, seed=seed + n)
                                       for n in ns]))
     return [lambda s=s: Request(s[0], s[1], s[2], coords=("xt", "xt_radial"), d12="2.5") for s in specs]
 
