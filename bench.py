@@ -315,7 +315,7 @@ def json_emission(runs, dt_step, blocks_step):
     bodies = BJ.poly_bodies(m, e, p, eps, threads=threads)
     dt = time.time() - t
     nbytes = sum(len(x) for x in bodies)
-    numbers = sum(x.count('",') + 1 for x in bodies)
+    numbers = sum(x.count(b'",') + 1 for x in bodies)
     rate = numbers / dt
     total = 0
     for g in runs:

@@ -44,12 +44,40 @@ struct Nat {
   }
 };
 
-// the 19 decimal digits of x < 10^19, zero padded
-void digits19(u64 x, char* o) {
-  for (int k = 18; k >= 0; --k) {
-    o[k] = (char)('0' + x % 10u);
-    x /= 10u;
+// two decimal digits of each value 0..99
+struct Pairs {
+  char d[200];
+  Pairs() {
+    for (int i = 0; i < 100; ++i) {
+      d[2 * i] = (char)('0' + i / 10);
+      d[2 * i + 1] = (char)('0' + i % 10);
+    }
   }
+};
+const Pairs PAIRS;
+
+// the 2k decimal digits of x < 10^(2k), zero padded, two at a time
+template <int K>
+inline void digits_pairs(u32 x, char* o) {
+  for (int k = K - 1; k >= 0; --k) {
+    const u32 q = x / 100u, r = x - 100u * q;
+    o[2 * k] = PAIRS.d[2 * r];
+    o[2 * k + 1] = PAIRS.d[2 * r + 1];
+    x = q;
+  }
+}
+
+// the 19 decimal digits of x < 10^19, zero padded: 10^19 = 10^11 * 10^8 splits x into a high part below 10^8
+// (then 10^8 = 10^8) and a low part below 10^11, each converted two digits at a time
+void digits19(u64 x, char* o) {
+  const u64 hi = x / 100000000000ull;            // < 10^8
+  const u64 lo = x - hi * 100000000000ull;        // < 10^11
+  digits_pairs<4>((u32)hi, o);                    // 8 digits
+  const u64 lh = lo / 1000000ull;                 // < 10^5
+  const u32 ll = (u32)(lo - lh * 1000000ull);     // < 10^6
+  o[8] = (char)('0' + (u32)lh / 10000u);          // 5 digits of lh
+  digits_pairs<2>((u32)lh % 10000u, o + 9);
+  digits_pairs<3>(ll, o + 13);                    // 6 digits
 }
 
 // n /= 10^19, returning the remainder

@@ -139,7 +139,7 @@ def test_poly_sizes_match_trimmed_length(L):
     assert sizes.tolist() == want
     for threads in (1, 5):
         got = BJ.poly_bodies(m.reshape(L, len(polys), D), e.reshape(len(polys), D), p, eps, threads=threads)
-        assert got == [body(q, p, eps) for q in padded]
+        assert got == [body(q, p, eps).encode() for q in padded]
 
 
 def test_derivs_text_same_from_bodies():
@@ -160,7 +160,7 @@ def test_select_and_errors():
     vals = [(3, -1), (5, -2), (0, EZ), (7, 0)]
     m, e = planes(vals, L)
     got = BJ.poly_bodies(m.reshape(L, 4, 1), e.reshape(4, 1), p, -900, select=np.array([3, 0, 2]))
-    assert got == ['"7"', '"1.5"', '"0"']
+    assert got == [b'"7"', b'"1.5"', b'"0"']
     buf = np.empty(4, dtype=np.uint8)
     end = np.empty(1, dtype=np.int64)
     with pytest.raises(_lib.B3DError, match="too small"):
