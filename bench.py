@@ -224,7 +224,7 @@ def pmc_traffic(workload):
 
 
 def roofline(timer, workload):
-    """The Cauchy step (k_cauchy_rows or k_cauchy, one launch per level), the kernel with the largest share of
+    """The Cauchy step (k_cauchy_groups, one launch per level), the kernel with the largest share of
     device time: algorithmic bytes per launch (gathered
     residues, h_inf entries and H entries, each counted once) over its average launch time, both from the
     timed region (HIP events recorded around every launch on the launch stream)"""
@@ -235,7 +235,7 @@ def roofline(timer, workload):
     avg_s = sum(ms) / len(ms) * 1e-3
     per_launch = nbytes / len(timer)
     achieved = per_launch / avg_s / 1e9
-    return {"kernel": "k_cauchy_rows<34> | k_cauchy<34>", "bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
+    return {"kernel": "k_cauchy_groups<34>", "bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
             "frac": achieved / 8000.0, "traffic": pmc_traffic(workload), "launches": len(timer),
             "avg_launch_ms": avg_s * 1e3, "algorithmic_bytes_per_launch": per_launch}
 

@@ -4,8 +4,9 @@
 //   k_cauchy   H[h, n] = h_inf[g(h), n] + sum_i x_i * 2 / (D'_h - D_i), x_i = res[col[g, i], n]:
 //              gathers the prime-pole residues straight from the slot buffer (no staging copy), aligns
 //              them per column, divides by the one-limb integer D' - D_i, sums exactly, rounds once and
-//              adds the h_inf entry (k_cauchy_rows when the rows, padded to whole waves, keep at least
-//              3/4 of the lanes busy: the group's data then sits in scalar registers; k_cauchy otherwise);
+//              adds the h_inf entry.  k_cauchy_groups with a wave table (a group's rows over consecutive
+//              lanes, the group's data in scalar registers); without one, k_cauchy_rows when the rows,
+//              padded to whole waves, keep at least 3/4 of the lanes busy, k_cauchy otherwise;
 //   k_pole_r   Y[t, b0, a, k] = sum_{b < SR} Ralign[R_t][b0][b] * H[hrow_t][a, b][k]
 //   k_pole_l   res[slot_t][a0, b0][k] = sum_{a < SL} QLalign[qL_t][a0][a] * Y[t, b0, a, k]
 //
@@ -48,6 +49,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L <= 34 ? 5
     mp::store(r, om, ops, t);
     oe[t] = er;
   }
+}
+
+// Group-uniform form: the rows of one group (which gather the same residue columns from the same h_inf row)
+// are laid out column by column over consecutive lanes, a wave at a time, padded only at the end of the group:
+// a wave may span several rows of its group, so only the row's position D' (one VGPR) varies across its
+// lanes, while the group's column list, divisor positions and plane bases are scalars.  Wave w covers
+// entries woff[w] + lane of the group whose rows start at wrow[w] (wcnt[w] rows, N columns each).
+template <int L>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L <= 34 ? 5 : 1))) void k_cauchy_groups(
+    const int32_t* __restrict__ rm, const i64* __restrict__ re, i64 rps, i64 N, const int32_t* __restrict__ hm,
+    const i64* __restrict__ he, i64 hps, const int64_t* __restrict__ ghrow, const int32_t* __restrict__ rg,
+    const int32_t* __restrict__ rd, const int64_t* __restrict__ col, const int32_t* __restrict__ cold,
+    const int32_t* __restrict__ ncol, int Kc, int32_t* __restrict__ om, i64* __restrict__ oe, i64 ops,
+    const int32_t* __restrict__ wrow, const int32_t* __restrict__ wcnt, const int32_t* __restrict__ woff, i64 nw) {
+  const unsigned lb = b3d_xcd_block(blockIdx.x, gridDim.x);   // a group's waves stay on one XCD's L2
+  const i64 w = (i64)lb * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (w >= nw) return;
+  const int h0 = __builtin_amdgcn_readfirstlane(wrow[w]);
+  const int cnt = __builtin_amdgcn_readfirstlane(wcnt[w]);
+  const i64 o = (i64)__builtin_amdgcn_readfirstlane(woff[w]) + (threadIdx.x & 63);
+  if (o >= (i64)cnt * N) return;
+  const int g = __builtin_amdgcn_readfirstlane(rg[h0]);
+  const i64 h = h0 + o / N, n = o % N;
+  u32 r[L];
+  const i64 er = mp::cauchy_entry<L, true>(rm, re, rps, N, n, col + (i64)g * Kc, cold + (i64)g * Kc, ncol[g], rd[h],
+                                           hm, he, hps, ghrow[g], r);
+  const i64 t = h * N + n;
+  mp::store(r, om, ops, t);
+  oe[t] = er;
 }
 
 // General form (rows that would leave many lanes idle if padded to whole waves): every index per lane.
@@ -121,10 +151,19 @@ extern "C" {
 int b3d_cauchy_level(int L, const int32_t* rm, const int64_t* re, int64_t rps, int64_t N, const int32_t* hm,
                      const int64_t* he, int64_t hps, const int64_t* ghrow, const int32_t* rg, const int32_t* rd,
                      int64_t nH, const int64_t* col, const int32_t* cold, const int32_t* ncol, int Kc, int32_t* om,
-                     int64_t* oe, int64_t ops, void* stream) {
+                     int64_t* oe, int64_t ops, const int32_t* wrow, const int32_t* wcnt, const int32_t* woff,
+                     int64_t nw, void* stream) {
   if (nH <= 0 || N <= 0) return 0;
   if (b3d::check_planes("b3d_cauchy_level", {rps, hps, ops})) return 2;
   hipStream_t s = (hipStream_t)stream;
+  if (wrow && wcnt && woff && nw > 0) {
+    if (nw > (int64_t(1) << 33)) {
+      b3d::set_error("b3d_cauchy_level: too many waves");
+      return 2;
+    }
+    B3D_DISPATCH_L(L, k_cauchy_groups<LL><<<dim3((unsigned)((nw + 3) / 4)), dim3(256), 0, s>>>(rm, re, rps, N, hm, he, hps, ghrow, rg, rd, col, cold, ncol, Kc, om, oe, ops, wrow, wcnt, woff, nw));
+    return b3d::check_launch("b3d_cauchy_level");
+  }
   // row-uniform kernel unless padding the rows to whole waves would idle more than a quarter of the lanes
   const int64_t W = (N + 63) / 64 * 64;
   if (4 * (W - N) <= W) {

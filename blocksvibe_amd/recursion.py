@@ -153,6 +153,31 @@ class Recursion:
         live = [lv for lv in self.levels if lv is not None]
         self.maxH = max((lv["nH"] for lv in live), default=1)
         self.maxT = max((lv["T"] for lv in live), default=1)
+        self._waves = {}
+
+    def wave_tables(self, N):
+        """per level, the Cauchy step's wave table for N columns per row (cached per N): each group's rows
+        (contiguous in the level's row order) laid out column-major over consecutive lanes, 64 lanes a wave,
+        padded at the end of each group -> (first row of the group, rows of the group, first entry) per wave"""
+        if N not in self._waves:
+            d, tabs = mpf.dev(), []
+            for lev in self.plan.levels:
+                if lev is None:
+                    tabs.append(None)
+                    continue
+                rg = lev.rg
+                starts = np.flatnonzero(np.r_[True, rg[1:] != rg[:-1]])
+                assert len(starts) == len(np.unique(rg)), "a level's rows must be grouped"
+                cnts = np.diff(np.r_[starts, len(rg)])
+                nwg = (cnts * N + 63) // 64
+                wrow = np.repeat(starts, nwg).astype(np.int32)
+                wcnt = np.repeat(cnts, nwg).astype(np.int32)
+                woff = ((np.arange(int(nwg.sum())) - np.repeat(np.cumsum(nwg) - nwg, nwg)) * 64).astype(np.int32)
+                assert cnts.max() * N < 1 << 31
+                tabs.append((torch.from_numpy(wrow).to(d), torch.from_numpy(wcnt).to(d), torch.from_numpy(woff).to(d),
+                             len(wrow)))
+            self._waves[N] = tabs
+        return self._waves[N]
 
     def run(self, h_inf, hook=None, prof=None, res=None, timer=None):
         """h_inf: MPF (2, order + 1, Jh, SL SR, B) (structure entry (a, b) at a SR + b).  Returns the residue
@@ -174,7 +199,8 @@ class Recursion:
         Y = mpf.zeros((self.maxT, SR, SL, B), L)
         st = mpf.stream_ptr()
         P = lambda t: t.data_ptr()
-        for lv in self.levels:
+        waves = self.wave_tables(N)
+        for lv, wt in zip(self.levels, waves):
             if lv is None:
                 continue
             t0 = time.time() if prof is not None else None
@@ -183,7 +209,7 @@ class Recursion:
                 ev0.record()
             _lib.call("b3d_cauchy_level", L, P(res.m), P(res.e), res.numel(), N, P(h.m), P(h.e), h.numel(),
                       P(lv["ghrow"]), P(lv["rg"]), P(lv["rd"]), lv["nH"], P(lv["col"]), P(lv["cold"]),
-                      P(lv["ncol"]), lv["Kc"], P(H.m), P(H.e), H.numel(), st)
+                      P(lv["ncol"]), lv["Kc"], P(H.m), P(H.e), H.numel(), P(wt[0]), P(wt[1]), P(wt[2]), wt[3], st)
             if timer is not None:
                 ev1 = torch.cuda.Event(enable_timing=True)
                 ev1.record()
