@@ -110,6 +110,8 @@ class Recursion:
     # pole-step products on int8 MFMA where at least 32 rows share a constant row (bit-identical to the VALU
     # path); BLOCKSVIBE_POLE_MMA=1 / 0 selects the default
     mma = os.environ.get("BLOCKSVIBE_POLE_MMA", "0") == "1"
+    # the Cauchy step's group-uniform kernel (per-level wave tables); False: the library's kernels without them
+    group_waves = True
 
     def _constants(self):
         plan, Sm, L = self.plan, self.plan.Smax, self.L
@@ -199,7 +201,7 @@ class Recursion:
         Y = mpf.zeros((self.maxT, SR, SL, B), L)
         st = mpf.stream_ptr()
         P = lambda t: t.data_ptr()
-        waves = self.wave_tables(N)
+        waves = self.wave_tables(N) if self.group_waves else [(None, None, None, 0)] * len(self.levels)
         for lv, wt in zip(self.levels, waves):
             if lv is None:
                 continue
@@ -209,7 +211,8 @@ class Recursion:
                 ev0.record()
             _lib.call("b3d_cauchy_level", L, P(res.m), P(res.e), res.numel(), N, P(h.m), P(h.e), h.numel(),
                       P(lv["ghrow"]), P(lv["rg"]), P(lv["rd"]), lv["nH"], P(lv["col"]), P(lv["cold"]),
-                      P(lv["ncol"]), lv["Kc"], P(H.m), P(H.e), H.numel(), P(wt[0]), P(wt[1]), P(wt[2]), wt[3], st)
+                      P(lv["ncol"]), lv["Kc"], P(H.m), P(H.e), H.numel(), *(P(x) if x is not None else 0 for x in wt[:3]),
+                      wt[3], st)
             if timer is not None:
                 ev1 = torch.cuda.Event(enable_timing=True)
                 ev1.record()

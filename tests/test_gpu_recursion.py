@@ -122,3 +122,25 @@ def test_recursion_bits_and_exact(case):
         for ti in range(len(slots)):
             for e in range(S2 * B):
                 assert vals[ti * S2 * B + e] == got[int(lev_n.t_slot[ti]) * S2 * B + e], ("reuse", m, ti, e)
+
+
+@pytest.mark.parametrize("case", [CASES[1], CASES[2], CASES[6], CASES[3]],
+                         ids=lambda c: f"{c['tj12']}-{c['tj43']}-B{c['B']}")
+def test_cauchy_kernels_agree(case):
+    """the group-uniform Cauchy kernel (wave tables) against the kernels used without them (rows padded to
+    whole waves at 63 and 64 columns, every index per lane at 75 and 4 columns), bit for bit"""
+    from blocksvibe_amd.plan import Plan
+    from blocksvibe_amd.recursion import Recursion
+    c = case
+    L, B, pp = c["L"], c["B"], c["pp"]
+    plan = Plan(c["tj12"], c["tj43"], c["tjmax"], c["order"], pp, reuse=True)
+    M, H = random_operator(plan, c["tj12"], c["tj43"], pp, c["order"], B, c["seed"])
+    h = h_tensor(H, plan, L)
+    print(f"cauchy kernels: {plan.S2 * B} columns per row", flush=True)
+    outs = []
+    for waves in (True, False):
+        rec = Recursion(plan, M, L)
+        rec.group_waves = waves
+        outs.append(rec.run(h))
+    a, b = outs
+    assert torch.equal(a.e, b.e) and torch.equal(a.m, b.m)
