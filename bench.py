@@ -315,20 +315,32 @@ def json_emission(runs, dt_step, blocks_step):
     bodies = BJ.poly_bodies(m, e, p, eps, threads=threads)
     dt = time.time() - t
     nbytes = sum(len(x) for x in bodies)
-    numbers = sum(x.count(b'",') + 1 for x in bodies)
+    numbers = sum(x.numbers() for x in bodies)
     rate = numbers / dt
+    # the same bodies made on the GPU from the device result, copied to the host
+    torch.cuda.synchronize()
+    t = time.time()
+    dbodies = BJ.poly_bodies_device(r.m.reshape(L, -1, D), r.e.reshape(-1, D), p, eps, select=sel)
+    torch.cuda.synchronize()
+    dtd = time.time() - t
+    rated = numbers / dtd
     total = 0
     for g in runs:
         for si, (sign, p0, der, out, ns, hs) in enumerate(g.subs):
             total += len(layout(g, si, range(NQ_PER_SIGN))) * der.for_order(LAMBDA - p0).Dout
-    est = total / rate
+    est, estd = total / rate, total / rated
     return {"sample": f"group {heavy.tj12},{heavy.tj43}, sign +1, first task's {TJMAX // 2 + 1} spin files: "
                       f"{len(sel)} polynomials x {D} coefficients, {numbers} kept numbers, "
                       f"{nbytes / 1e6:.1f} MB of list bodies",
-            "threads": threads, "seconds": dt, "numbers_per_s": rate, "mb_per_s": nbytes / dt / 1e6,
-            "step_numbers_upper_bound": total, "step_emission_s_estimate": est,
-            "blocks_per_s_with_json_overlapped": blocks_step / max(dt_step, est),
-            "note": "emission on host threads while the GPU runs the next groups; file writes not counted"}
+            "host": {"threads": threads, "seconds": dt, "numbers_per_s": rate, "mb_per_s": nbytes / dt / 1e6,
+                     "step_emission_s_estimate": est,
+                     "blocks_per_s_with_json_overlapped": blocks_step / max(dt_step, est)},
+            "gpu": {"seconds": dtd, "numbers_per_s": rated, "mb_per_s": nbytes / dtd / 1e6,
+                    "identical_to_host": dbodies == bodies, "step_emission_s_estimate": estd,
+                    "blocks_per_s_with_json_serial": blocks_step / (dt_step + estd)},
+            "step_numbers_upper_bound": total,
+            "note": "host: formatting on host threads while the GPU runs the next groups; gpu: text made on the "
+                    "GPU after each group and copied to the host (serial with the chain); file writes not counted"}
 
 
 def main():

@@ -14,7 +14,8 @@ P, I64, I32, SZ = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_size_t
 _C = {"const int32_t*": P, "int32_t*": P, "const int64_t*": P, "int64_t*": P, "void*": P, "const void*": P,
       "const int8_t*": P, "int8_t*": P, "const uint32_t*": P, "uint32_t*": P, "const uint16_t*": P,
       "uint16_t*": P, "const double*": P, "double*": P, "int64_t": I64, "int": I32, "size_t": SZ,
-      "int*": P, "float*": P, "const float*": P, "char*": P}
+      "int*": P, "float*": P, "const float*": P, "char*": P,
+      "uint8_t*": P, "const uint8_t*": P}
 
 
 class B3DError(RuntimeError):

@@ -379,9 +379,10 @@ def run_group(tasks, timers):
 
 
 def gpu_group(tasks, q4s, sfile, mfile, timers):
-    """the GPU chain for one group -> {(q4, sign): (spins, pp, {(coord, d12): (m, e) host arrays (L, Jo, 2, S2,
-    outputs, Dmax) / (Jo, 2, S2, outputs, Dmax)}, {(coord, d12): outputs list}, SR)} with S2 = SL SR structure
-    entries, entry (a, b) at a SR + b"""
+    """the GPU chain for one group, writing every task's spin files (write_task_files) -> results for
+    write_outputs: {(q4, sign): (spins, pp, {(coord, d12): (m, e) host arrays (L, Jo, 2, S2, outputs, Dmax) /
+    (Jo, 2, S2, outputs, Dmax)}, {(coord, d12): outputs list}, SR)} with S2 = SL SR structure entries, entry
+    (a, b) at a SR + b; empty, since the files are already written"""
     import torch
     from . import mpf
     from . import setup as SU
@@ -399,7 +400,7 @@ def gpu_group(tasks, q4s, sfile, mfile, timers):
         sub.setdefault((st.pp, st.sign), []).append(st)
     torch.cuda.synchronize()
     timers.append((".main.gpu.read_setup", time.time() - t0, 1))
-    results = {}
+    results, write_s = {}, 0.0
     for (pp, sign), sts in sub.items():
         keys = {(s.q4, s.sign) for s in sts}
         ts = [t for t in tasks if t.sign == sign and (tuple(t.q4), t.sign) in keys]
@@ -430,26 +431,92 @@ def gpu_group(tasks, q4s, sfile, mfile, timers):
                 rec.run(h, hook=dn.accumulate)
                 nums.append(dn.finish(h))
                 del h, hs
-            for (c, d), o in outs.items():
-                r = o.apply_per_order(nums)                     # (Jo, 2, S2, nq, outputs, Dmax)
-                m = r.m.cpu().numpy()
-                e = r.e.cpu().numpy()
-                for q, st in enumerate(ch):
-                    ent = results.setdefault((st.q4, st.sign), (tjs, pp, {}, {}, SR))
-                    ent[2][(c, d)] = (m[:, :, :, :, q], e[:, :, :, q])
-                    ent[3][(c, d)] = o.outputs
+            rs = {cd: o.apply_per_order(nums) for cd, o in outs.items()}   # (Jo, 2, S2, nq, outputs, Dmax)
+            del nums
+            tw = time.time()
+            for q, st in enumerate(ch):
+                # every spin file of the tasks of this structure, their numbers formatted on the GPU
+                mine = [t for t in ts if (tuple(t.q4), t.sign) == (st.q4, st.sign)]
+                write_task_files(mine, tjs, pp, plan, rs, {cd: o.outputs for cd, o in outs.items()}, q)
+            write_s += time.time() - tw
+            del rs
         torch.cuda.synchronize()
         t3 = time.time()
-        timers.append((".main.gpu.chain", t3 - t2, len(ns) * len(chunks)))
+        timers.append((".main.gpu.chain", t3 - t2 - write_s, len(ns) * len(chunks)))
         log(f"pp {pp} sign {sign}: {len(sts)} structure(s) x {len(ns)} lambda order(s), batch <= {bmax}: "
             f"constants {t2 - t1:.2f} s, chain {t3 - t2:.2f} s")
+    timers.append((".main.gpu.write_output", write_s, 1))
     return results
 
 
+def write_task_files(mine, tjs, pp, plan, rs, outputs, q):
+    """the spin files of the CLI tasks `mine` (one four-point structure and sign: column q of the write_output
+    results rs[(coord, d12)] on the device, (Jo, 2, SL SR, nq, outputs, Dmax)): the polynomials each file
+    holds, as JSON list bodies made on the GPU (b3djson.poly_bodies_device), then the files"""
+    import torch
+    from . import b3djson as BJ
+    from . import structure as S
+    if not mine:
+        return
+    ps = {BJ.digits10(t.precision) for t in mine}
+    SR, S2 = plan.SRm, plan.S2
+    for p in ps:
+        eps = BJ.trim_exponent(p)
+        need = sorted({tj for t in mine if BJ.digits10(t.precision) == p for tj in t.two_js_internal})
+        bodies = {}
+        for cd, r in rs.items():
+            if not any(cd[0] in t.coordinates and cd[1] == t.delta_1_plus_2 for t in mine):
+                continue
+            Jo, _, _, nq, no, D = r.e.shape
+            sel, spans = [], {}
+            for tj in need:
+                jo = tjs.index(tj)
+                structs = [(len(S.so3(pp[k][0], plan.tj12, tj)), len(S.so3(pp[k][1], plan.tj43, tj))) for k in range(2)]
+                a0 = len(sel)
+                for k in range(2):
+                    for a in range(structs[k][0]):
+                        for b in range(structs[k][1]):
+                            base = ((jo * 2 + k) * S2 + a * SR + b) * nq + q
+                            sel += [base * no + o for o in range(no)]
+                spans[tj] = (a0, len(sel), structs)
+            got = BJ.poly_bodies_device(r.m.reshape(r.m.shape[0], -1, D), r.e.reshape(-1, D), p, eps,
+                                        select=torch.tensor(sel, dtype=torch.int64, device=r.e.device))
+            bodies[cd] = {tj: (got[a0:a1], structs) for tj, (a0, a1, structs) in spans.items()}
+        for t in mine:
+            if BJ.digits10(t.precision) != p:
+                continue
+            for tj in t.two_js_internal:
+                coord_rows, structs = [], None
+                for c in [c for c in BJ.COORD_ORDER if c in t.coordinates]:
+                    cd = (c, t.delta_1_plus_2)
+                    polys, structs = bodies[cd][tj]
+                    no = len(outputs[cd])
+                    rows_k, i = [], 0
+                    for k in range(2):
+                        rows_a = []
+                        for a in range(structs[k][0]):
+                            rows_b = []
+                            for b in range(structs[k][1]):
+                                rows = {}
+                                for o, (n, _) in enumerate(outputs[cd]):
+                                    rows.setdefault(n, []).append(polys[i])
+                                    i += 1
+                                rows_b.append([rows[n] for n in sorted(rows)])
+                            rows_a.append(rows_b)
+                        rows_k.append(rows_a)
+                    coord_rows.append(rows_k)
+                path = t.path(tj / 2.0)
+                os.makedirs(os.path.dirname(path) or ".", exist_ok=True)
+                BJ.write_spin(path, t, tj, pp, structs, coord_rows)
+
+
 def write_outputs(tasks, results):
-    """all spin files of all tasks (host data only): numbers formatted by the library's host formatter on up
-    to num_threads threads, one spin file after another"""
-    jobs = [(i, tj) for i, t in enumerate(tasks) for tj in t.two_js_internal]
+    """the spin files of the tasks that have host results (the GPU path writes its files itself, from bodies
+    made on the GPU): numbers formatted by the library's host formatter on up to num_threads threads, one
+    spin file after another"""
+    jobs = [(i, tj) for i, t in enumerate(tasks) if (tuple(t.q4), t.sign) in results for tj in t.two_js_internal]
+    if not jobs:
+        return
     threads = max(1, min(max(t.num_threads for t in tasks), os.cpu_count() or 1))
     for j in jobs:
         _write_one(tasks, results, j, threads)

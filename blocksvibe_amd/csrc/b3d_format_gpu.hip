@@ -1,0 +1,282 @@
+// libb3dhip: the JSON writer's decimal text on the GPU (the same text as b3d_format_polys in b3d_format.hip,
+// reference blocksvibe/b3djson.py:30-68 fmt_big / fmt_gmpy and :150-177 derivs_text).  Three kernels:
+//
+//   k_fmt_sizes   per polynomial: its kept length (the truncation of negligible top coefficients, exact
+//                 comparisons), 0 for an all-zero polynomial
+//   k_fmt_text    per element (a kept coefficient, or the "0" of an all-zero polynomial): the %.{p}g text of
+//                 the exact value.  The fraction is held as FW 32-bit limbs with the binary point above the top
+//                 limb (the integer part shifted out of the window); each
+//                 multiplication by 10^9 carries the next nine digits out of the top limb; the integer part
+//                 (the mantissa shifted right) is converted by repeated division by 10^9.  Digits go to a
+//                 per-element staging row, are rounded half to even with a sticky flag, and the text is laid
+//                 out from them.  Values outside that range (more than 32 FW fraction bits, exponent >= 0)
+//                 get length -1 and are formatted by the host formatter.
+//   k_fmt_pack    per element: '"' text '"' and ",\n" unless it ends its polynomial, at its offset in the
+//                 packed bodies (offsets: an exclusive prefix sum of the piece lengths)
+//
+// Staging rows and texts are stored byte-transposed (byte c of element k at c * n + k), so a wave's lanes
+// write consecutive bytes.
+#include "common.hpp"
+#include "mp.hpp"
+#include "../../include/b3dhip.h"
+
+namespace {
+
+template <int L>
+struct Fmt {
+  static constexpr int FW = L + 16;   // fraction limbs: values down to 2^(32 L - 2 - 32 FW)
+};
+
+// |x| 2^ex < |y| 2^ey for nonzero two's complement entries i, j of the planes (ey += eps)
+template <int L>
+__device__ __forceinline__ bool lt_abs(const int32_t* m, i64 ps, i64 i, i64 ei, i64 j, i64 ej) {
+  u32 a[L], b[L];
+  mp::load(a, m, ps, i);
+  mp::load(b, m, ps, j);
+  mp::to_mag(a);
+  mp::to_mag(b);
+  const int ba = mp::bitlen(a), bb = mp::bitlen(b);
+  const i64 ta = ba + ei, tb = bb + ej;
+  if (ta != tb) return ta < tb;
+  // same top bit: put both on the larger bit length (the shift is below 32 L)
+  if (ba < bb) mp::shl(a, bb - ba);
+  if (bb < ba) mp::shl(b, ba - bb);
+  for (int k = L - 1; k >= 0; --k)
+    if (a[k] != b[k]) return a[k] < b[k];
+  return false;
+}
+
+}  // namespace
+
+template <int L>
+__global__ __launch_bounds__(256) void k_fmt_sizes(const int32_t* __restrict__ m, const i64* __restrict__ e, i64 ps,
+                                                   const int64_t* __restrict__ pbase, i64 npoly, int deg, i64 eps,
+                                                   int32_t* __restrict__ sizes) {
+  for (i64 q = blockIdx.x * (i64)blockDim.x + threadIdx.x; q < npoly; q += (i64)gridDim.x * blockDim.x) {
+    const i64 base = pbase[q];
+    int size = 0;
+    i64 mx = -1, emx = 0;
+    for (int d = deg - 1; d >= 0; --d) {
+      const i64 i = base + d;
+      const i64 ei = e[i];
+      if (ei == B3D_EZ) continue;   // canonical zero: 0 < 2^eps 0 is false
+      if (mx < 0 || lt_abs<L>(m, ps, mx, emx, i, ei + eps)) size = d + 1;
+      if (mx < 0 || lt_abs<L>(m, ps, mx, emx, i, ei)) {
+        mx = i;
+        emx = ei;
+      }
+    }
+    sizes[q] = size;
+  }
+}
+
+template <int L>
+__global__ __launch_bounds__(256) void k_fmt_text(const int32_t* __restrict__ m, const i64* __restrict__ e, i64 ps,
+                                                  const int64_t* __restrict__ ent, i64 n, int p, uint8_t* __restrict__ dg,
+                                                  uint8_t* __restrict__ tx, int32_t* __restrict__ len) {
+  constexpr int FW = Fmt<L>::FW;
+  for (i64 k = blockIdx.x * (i64)blockDim.x + threadIdx.x; k < n; k += (i64)gridDim.x * blockDim.x) {
+    const i64 i = ent[k];
+    auto put = [&](int c, char v) { tx[(i64)c * n + k] = (uint8_t)v; };
+    auto dput = [&](int c, int v) { dg[(i64)c * n + k] = (uint8_t)v; };
+    auto dget = [&](int c) -> int { return dg[(i64)c * n + k]; };
+    if (i < 0 || e[i] == B3D_EZ) {   // the "0" of an all-zero polynomial, or a zero coefficient
+      put(0, '0');
+      len[k] = 1;
+      continue;
+    }
+    u32 x[L];
+    mp::load(x, m, ps, i);
+    const bool neg = mp::to_mag(x) != 0u;
+    const i64 ex = e[i];
+    const int bl = mp::bitlen(x);
+    const i64 fb = -ex;   // fraction bits
+    if (ex >= 0 || fb > 32 * FW) {
+      len[k] = -1;          // left to the host formatter
+      continue;
+    }
+    // digits: nd significant digits staged, X the decimal exponent of the first
+    int nd = 0, X = -1;
+    if (bl > fb) {
+      // integer part (the mantissa shifted right by fb, at most L limbs) by repeated division by 10^9; the
+      // digits come least significant first (the top chunk without its leading zeros) and are then reversed
+      u32 t[L];
+#pragma unroll
+      for (int q = 0; q < L; ++q) t[q] = x[q];
+      mp::shr(t, (int)fb);
+      bool more = true;
+      while (more) {
+        u64 rem = 0;
+        u32 any = 0u;
+#pragma unroll
+        for (int q = L - 1; q >= 0; --q) {
+          const u64 cur = (rem << 32) | t[q];
+          const u64 qq = cur / 1000000000ull;
+          t[q] = (u32)qq;
+          rem = cur - qq * 1000000000ull;
+          any |= t[q];
+        }
+        more = any != 0u;
+        u32 chunk = (u32)rem;
+        for (int r = 0; r < 9; ++r) {
+          if (!more && chunk == 0u) break;
+          dput(nd++, (int)(chunk % 10u));
+          chunk /= 10u;
+        }
+      }
+      for (int a = 0, b = nd - 1; a < b; ++a, --b) {
+        const int va = dget(a), vb = dget(b);
+        dput(a, vb);
+        dput(b, va);
+      }
+      X = nd - 1;
+    }
+    // the fraction shifted to the top of FW limbs (the integer part's bits are shifted out of the window)
+    u32 F[FW];
+#pragma unroll
+    for (int q = 0; q < FW; ++q) F[q] = q < L ? x[q] : 0u;
+    mp::shl(F, (int)(32 * FW - fb));
+    bool nz = false;
+#pragma unroll
+    for (int q = 0; q < FW; ++q) nz |= F[q] != 0u;
+    while (nd < p + 1 && nz) {
+      u32 carry = 0;
+#pragma unroll
+      for (int q = 0; q < FW; ++q) {
+        const u64 v = (u64)F[q] * 1000000000u + carry;
+        F[q] = (u32)v;
+        carry = (u32)(v >> 32);
+      }
+      nz = false;
+#pragma unroll
+      for (int q = 0; q < FW; ++q) nz |= F[q] != 0u;
+      u32 chunk = carry, dig[9];   // nine digits (constant divisors: multiply-high sequences)
+#pragma unroll
+      for (int q = 8; q >= 0; --q) {
+        dig[q] = chunk % 10u;
+        chunk /= 10u;
+      }
+#pragma unroll
+      for (int q = 0; q < 9; ++q) {
+        if (nd == 0 && dig[q] == 0u) {
+          --X;
+          continue;
+        }
+        dput(nd++, (int)dig[q]);
+      }
+    }
+    const bool sticky = nz;
+    // round to p digits, half to even
+    if (nd > p) {
+      const int r = dget(p);
+      bool below = sticky;
+      for (int q = p + 1; q < nd && !below; ++q) below = dget(q) != 0;
+      nd = p;
+      if (r > 5 || (r == 5 && (below || (dget(p - 1) & 1)))) {
+        int q = p - 1;
+        while (q >= 0 && dget(q) == 9) dput(q--, 0);
+        if (q >= 0) {
+          dput(q, dget(q) + 1);
+        } else {   // 99..9 -> 100..0: one more decimal place
+          dput(0, 1);
+          ++X;
+        }
+      }
+    }
+    while (nd > 1 && dget(nd - 1) == 0) --nd;
+    // layout (C's %g): d.ddde+XX outside -4 <= X < p, fixed notation inside
+    int c = 0;
+    if (neg) put(c++, '-');
+    if (X < -4 || X >= p) {
+      put(c++, (char)('0' + dget(0)));
+      if (nd > 1) {
+        put(c++, '.');
+        for (int q = 1; q < nd; ++q) put(c++, (char)('0' + dget(q)));
+      }
+      put(c++, 'e');
+      put(c++, X < 0 ? '-' : '+');
+      int ax = X < 0 ? -X : X;
+      char tmp[12];
+      int w = 0;
+      do {
+        tmp[w++] = (char)('0' + ax % 10);
+        ax /= 10;
+      } while (ax);
+      if (w < 2) tmp[w++] = '0';
+      while (w) put(c++, tmp[--w]);
+    } else if (X >= 0) {
+      for (int q = 0; q <= X; ++q) put(c++, q < nd ? (char)('0' + dget(q)) : '0');
+      if (nd > X + 1) {
+        put(c++, '.');
+        for (int q = X + 1; q < nd; ++q) put(c++, (char)('0' + dget(q)));
+      }
+    } else {
+      put(c++, '0');
+      put(c++, '.');
+      for (int q = 0; q < -1 - X; ++q) put(c++, '0');
+      for (int q = 0; q < nd; ++q) put(c++, (char)('0' + dget(q)));
+    }
+    len[k] = c;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_fmt_pack(const uint8_t* __restrict__ tx, const int32_t* __restrict__ len,
+                                                  const uint8_t* __restrict__ last, const int64_t* __restrict__ off,
+                                                  i64 n, uint8_t* __restrict__ out) {
+  for (i64 k = blockIdx.x * (i64)blockDim.x + threadIdx.x; k < n; k += (i64)gridDim.x * blockDim.x) {
+    uint8_t* o = out + off[k];
+    const int l = len[k];
+    *o++ = '"';
+    for (int c = 0; c < l; ++c) *o++ = tx[(i64)c * n + k];
+    *o++ = '"';
+    if (!last[k]) {
+      *o++ = ',';
+      *o++ = '\n';
+    }
+  }
+}
+
+static inline unsigned grid_for(int64_t n, int th) {
+  int64_t b = (n + th - 1) / th;
+  if (b > (1 << 22)) b = 1 << 22;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+extern "C" {
+
+int b3d_fmt_sizes(int L, const int32_t* m, const int64_t* e, int64_t ps, const int64_t* pbase, int64_t npoly, int deg,
+                  int64_t eps_exp, int32_t* sizes, void* stream) {
+  if (npoly <= 0) return 0;
+  if (deg < 1) {
+    b3d::set_error("b3d_fmt_sizes: need deg >= 1");
+    return 2;
+  }
+  if (b3d::check_planes("b3d_fmt_sizes", {ps})) return 2;
+  hipStream_t s = (hipStream_t)stream;
+  B3D_DISPATCH_L(L, k_fmt_sizes<LL><<<dim3(grid_for(npoly, 256)), dim3(256), 0, s>>>(m, e, ps, pbase, npoly, deg, eps_exp, sizes));
+  return b3d::check_launch("b3d_fmt_sizes");
+}
+
+int b3d_fmt_text(int L, const int32_t* m, const int64_t* e, int64_t ps, const int64_t* ent, int64_t n, int p,
+                 uint8_t* digits, uint8_t* text, int32_t* len, void* stream) {
+  if (n <= 0) return 0;
+  if (p < 1 || p > 2000) {
+    b3d::set_error("b3d_fmt_text: need 1 <= p <= 2000");
+    return 2;
+  }
+  if (b3d::check_planes("b3d_fmt_text", {ps})) return 2;
+  hipStream_t s = (hipStream_t)stream;
+  B3D_DISPATCH_L(L, k_fmt_text<LL><<<dim3(grid_for(n, 256)), dim3(256), 0, s>>>(m, e, ps, ent, n, p, digits, text, len));
+  return b3d::check_launch("b3d_fmt_text");
+}
+
+int b3d_fmt_pack(const uint8_t* text, const int32_t* len, const uint8_t* last, const int64_t* off, int64_t n,
+                 uint8_t* out, void* stream) {
+  if (n <= 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  k_fmt_pack<<<dim3(grid_for(n, 256)), dim3(256), 0, s>>>(text, len, last, off, n, out);
+  return b3d::check_launch("b3d_fmt_pack");
+}
+
+}  // extern "C"

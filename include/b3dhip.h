@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-#define B3D_ABI_VERSION 10
+#define B3D_ABI_VERSION 11
 
 /* Last error message of the calling thread (empty if none). */
 const char* b3d_last_error(void);
@@ -191,6 +191,22 @@ int b3d_poly_sizes(int L, const int32_t* m, const int64_t* e, int64_t ps, int64_
  * concatenated into out (cap bytes); body q ends at end[q].  Work is split over threads by polynomial. */
 int b3d_format_polys(int L, const int32_t* m, const int64_t* e, int64_t ps, int64_t npoly, int deg, int p,
                      int64_t eps_exp, char* out, int64_t cap, int64_t* end, int threads);
+
+/* ---- output emission on the GPU (device arrays; the same text as b3d_format_polys) ---- */
+
+/* Kept length of each polynomial q < npoly whose deg coefficients are entries pbase[q] .. pbase[q] + deg - 1
+ * of the device planes m / e (plane stride ps): as b3d_poly_sizes. */
+int b3d_fmt_sizes(int L, const int32_t* m, const int64_t* e, int64_t ps, const int64_t* pbase, int64_t npoly, int deg,
+                  int64_t eps_exp, int32_t* sizes, void* stream);
+/* Text of n elements: element k is entry ent[k] of the planes (ent[k] < 0: the text "0"), formatted as
+ * b3d_format_numbers does, byte c of its text at text[c * n + k] (text: n * (p + 16) bytes), its length at
+ * len[k]; len[k] = -1 for a value outside the GPU's range (exponent >= 0, or fraction bits beyond
+ * 32 (L + 16)), to be formatted on the host.  digits: staging of n * (p + 10 L + 32) bytes. */
+int b3d_fmt_text(int L, const int32_t* m, const int64_t* e, int64_t ps, const int64_t* ent, int64_t n, int p,
+                 uint8_t* digits, uint8_t* text, int32_t* len, void* stream);
+/* Packed bodies: element k as '"' text '"' followed by ",\n" unless last[k], at out + off[k]. */
+int b3d_fmt_pack(const uint8_t* text, const int32_t* len, const uint8_t* last, const int64_t* off, int64_t n,
+                 uint8_t* out, void* stream);
 
 #ifdef __cplusplus
 }
