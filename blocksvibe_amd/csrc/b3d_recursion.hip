@@ -101,48 +101,96 @@ __global__ __launch_bounds__(256) void k_cauchy(const int32_t* __restrict__ rm, 
   }
 }
 
+// The pole step's two products, a workgroup per block of columns: a column is the SR (R product) or SL (q L
+// product) X entries one output row combines; every output of the column shares them, so the workgroup
+// aligns each column once (exponent from its nonzero terms), converts each term once into 28-bit limbs in
+// LDS (thread (column, s) stages term s), and every thread then forms its output's dot product from LDS
+// (mp::small_dot_staged, the same bits as mp::small_dot).  Threads: s fastest within a column.
 template <int L>
-__global__ __launch_bounds__(256) void k_pole_r(const int32_t* __restrict__ Hm, const i64* __restrict__ He, i64 hps,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L <= 34 ? 2 : 1))) void k_pole_r(const int32_t* __restrict__ Hm, const i64* __restrict__ He, i64 hps,
                                                 const int64_t* __restrict__ thrw, const int32_t* __restrict__ tR,
                                                 i64 T, int SL, int SR, int S, i64 B, const int32_t* __restrict__ R28,
                                                 const uint32_t* __restrict__ Rsg, const i64* __restrict__ Re,
                                                 i64 rps, int32_t* __restrict__ Ym, i64* __restrict__ Ye, i64 yps) {
-  const i64 LB = (i64)SL * B, NE = (i64)SR * LB;   // entries of one task's Y (b0, a, k)
+  constexpr int NL = mp::R28<L>::NL;
+  extern __shared__ int32_t st[];
+  const int CPB = 256 / SR;                   // columns (task, a, k) per block, SR outputs b0 each
+  const i64 ncol = T * SL * B, NE = (i64)SR * SL * B;
+  const int lc = threadIdx.x / SR, b0 = threadIdx.x % SR;
   const unsigned lb = b3d_xcd_block(blockIdx.x, gridDim.x);   // a task's outputs stay on one XCD's L2
-  for (i64 t = lb * (i64)blockDim.x + threadIdx.x; t < T * NE; t += (i64)gridDim.x * blockDim.x) {
-    const i64 k = t % B;
-    const i64 a = (t / B) % SL;
-    const i64 b0 = (t / LB) % SR;
-    const i64 task = t / NE;
-    const i64 r = tR[task];
-    const i64 xbase = thrw[task] * NE + a * SR * B + k;  // H[hrow][a*SR + b][k], b stride B
-    u32 out[L];
-    const i64 e = mp::small_dot<L>(R28, Rsg, rps, (r * S + b0) * S, Re[r * S + b0], Hm, He, hps, xbase, B, SR, out);
-    mp::store(out, Ym, yps, t);  // (task, b0, a, k)
-    Ye[t] = e;
+  for (i64 c0 = (i64)lb * CPB; c0 < ncol; c0 += (i64)gridDim.x * CPB) {
+    const i64 col = c0 + lc;
+    const bool act = lc < CPB && col < ncol;
+    i64 task = 0, a = 0, k = 0, xbase = 0, E = B3D_EZ;
+    if (act) {
+      k = col % B;
+      a = (col / B) % SL;
+      task = col / ((i64)SL * B);
+      xbase = thrw[task] * NE + a * SR * B + k;   // H[hrow][a*SR + b][k], b stride B
+      for (int j = 0; j < SR; ++j) {
+        const i64 idx = xbase + j * B;
+        if (mp::load_limb(Hm, hps, L - 1, idx) != 0u) {
+          const i64 ej = He[idx];
+          E = ej > E ? ej : E;
+        }
+      }
+      mp::stage_term<L>(Hm, He, hps, xbase + b0 * B, E, st + threadIdx.x * (NL + 1));
+    }
+    __syncthreads();
+    if (act) {
+      const i64 r = tR[task];
+      u32 out[L];
+      const i64 e = mp::small_dot_staged<L>(R28, Rsg, rps, (r * S + b0) * S, Re[r * S + b0],
+                                            st + lc * SR * (NL + 1), SR, E, out);
+      const i64 t = ((task * SR + b0) * SL + a) * B + k;   // Y (task, b0, a, k)
+      mp::store(out, Ym, yps, t);
+      Ye[t] = e;
+    }
+    __syncthreads();
   }
 }
 
 template <int L>
-__global__ __launch_bounds__(256) void k_pole_l(const int32_t* __restrict__ Ym, const i64* __restrict__ Ye, i64 yps,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L <= 34 ? 2 : 1))) void k_pole_l(const int32_t* __restrict__ Ym, const i64* __restrict__ Ye, i64 yps,
                                                 const int32_t* __restrict__ tQL, const int64_t* __restrict__ tslot,
                                                 i64 T, int SL, int SR, int S, i64 B, const int32_t* __restrict__ Q28,
                                                 const uint32_t* __restrict__ Qsg, const i64* __restrict__ Qe, i64 qps,
                                                 int32_t* __restrict__ rm, i64* __restrict__ re, i64 rps) {
-  const i64 RB = (i64)SR * B, NE = (i64)SL * RB;   // entries of one task's result (a0, b0, k)
-  const unsigned lb = b3d_xcd_block(blockIdx.x, gridDim.x);   // a task's outputs stay on one XCD's L2
-  for (i64 t = lb * (i64)blockDim.x + threadIdx.x; t < T * NE; t += (i64)gridDim.x * blockDim.x) {
-    const i64 k = t % B;
-    const i64 b0 = (t / B) % SR;
-    const i64 a0 = (t / RB) % SL;
-    const i64 task = t / NE;
-    const i64 q = tQL[task];
-    const i64 xbase = (task * SR + b0) * SL * B + k;  // Y[task][b0][a][k], a stride B
-    u32 out[L];
-    const i64 e = mp::small_dot<L>(Q28, Qsg, qps, (q * S + a0) * S, Qe[q * S + a0], Ym, Ye, yps, xbase, B, SL, out);
-    const i64 o = tslot[task] * NE + (a0 * SR + b0) * B + k;
-    mp::store(out, rm, rps, o);
-    re[o] = e;
+  constexpr int NL = mp::R28<L>::NL;
+  extern __shared__ int32_t st[];
+  const int CPB = 256 / SL;                   // columns (task, b0, k) per block, SL outputs a0 each
+  const i64 ncol = T * SR * B, NE = (i64)SL * SR * B;
+  const int lc = threadIdx.x / SL, a0 = threadIdx.x % SL;
+  const unsigned lb = b3d_xcd_block(blockIdx.x, gridDim.x);
+  for (i64 c0 = (i64)lb * CPB; c0 < ncol; c0 += (i64)gridDim.x * CPB) {
+    const i64 col = c0 + lc;
+    const bool act = lc < CPB && col < ncol;
+    i64 task = 0, b0 = 0, k = 0, xbase = 0, E = B3D_EZ;
+    if (act) {
+      k = col % B;
+      b0 = (col / B) % SR;
+      task = col / ((i64)SR * B);
+      xbase = (task * SR + b0) * SL * B + k;   // Y[task][b0][a][k], a stride B
+      for (int j = 0; j < SL; ++j) {
+        const i64 idx = xbase + j * B;
+        if (mp::load_limb(Ym, yps, L - 1, idx) != 0u) {
+          const i64 ej = Ye[idx];
+          E = ej > E ? ej : E;
+        }
+      }
+      mp::stage_term<L>(Ym, Ye, yps, xbase + a0 * B, E, st + threadIdx.x * (NL + 1));
+    }
+    __syncthreads();
+    if (act) {
+      const i64 q = tQL[task];
+      u32 out[L];
+      const i64 e = mp::small_dot_staged<L>(Q28, Qsg, qps, (q * S + a0) * S, Qe[q * S + a0],
+                                            st + lc * SL * (NL + 1), SL, E, out);
+      const i64 o = tslot[task] * NE + (a0 * SR + b0) * B + k;
+      mp::store(out, rm, rps, o);
+      re[o] = e;
+    }
+    __syncthreads();
   }
 }
 
@@ -192,10 +240,13 @@ int b3d_pole_level(int L, int SL, int SR, int S, int64_t B, int64_t T, const int
     if (b3d::pole_mma(L, Hm, He, hps, thrw, tR, Rd8, Rcorr, Re, nullptr, T, S, B, Ym, Ye, yps, s)) return 1;
     return b3d::pole_mma(L, Ym, Ye, yps, nullptr, tQL, Qd8, Qcorr, Qe, tslot, T, S, B, rm, re, rps, s);
   }
-  const int64_t n = T * SL * SR * B;
-  B3D_DISPATCH_L(L, k_pole_r<LL><<<dim3(grid_for(n, 256)), dim3(256), 0, s>>>(Hm, He, hps, thrw, tR, T, SL, SR, S, B, R28, Rsg, Re, rtps, Ym, Ye, yps));
+  // one block per 256 / SR (R product) or 256 / SL (q L product) columns, each staging its terms in LDS
+  const int nl = b3d_limbs28(L);
+  const size_t lds = 256 * (size_t)(nl + 1) * sizeof(int32_t);
+  const int64_t colR = T * SL * B, colL = T * SR * B;
+  B3D_DISPATCH_L(L, k_pole_r<LL><<<dim3(grid_for(colR * SR, 256 / SR * SR)), dim3(256), lds, s>>>(Hm, He, hps, thrw, tR, T, SL, SR, S, B, R28, Rsg, Re, rtps, Ym, Ye, yps));
   if (b3d::check_launch("b3d_pole_level (R)")) return 1;
-  B3D_DISPATCH_L(L, k_pole_l<LL><<<dim3(grid_for(n, 256)), dim3(256), 0, s>>>(Ym, Ye, yps, tQL, tslot, T, SL, SR, S, B, Q28, Qsg, Qe, qps, rm, re, rps));
+  B3D_DISPATCH_L(L, k_pole_l<LL><<<dim3(grid_for(colL * SL, 256 / SL * SL)), dim3(256), lds, s>>>(Ym, Ye, yps, tQL, tslot, T, SL, SR, S, B, Q28, Qsg, Qe, qps, rm, re, rps));
   return b3d::check_launch("b3d_pole_level (L)");
 }
 

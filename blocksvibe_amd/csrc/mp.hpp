@@ -625,6 +625,80 @@ __device__ __forceinline__ i64 small_dot(const int32_t* __restrict__ a28, const 
   return mp::norm<L, NWH>(U, ea + E + 28 * C0, out);
 }
 
+// small_dot with its X terms already aligned and converted (by the workgroup, into LDS): term j at
+// st + j (NL + 1): NL signed 28-bit limbs (the sign of X folded in) and a flag (0: a zero term), E the terms'
+// common exponent.  The constant's sign is folded into its limbs here.  Same rounding as small_dot (truncated
+// product, certainty test, full product where it fails): the same bits.
+template <int L>
+__device__ __forceinline__ i64 small_dot_staged(const int32_t* __restrict__ a28, const uint32_t* __restrict__ asg,
+                                                i64 aps, i64 abase, i64 ea, const int32_t* st, int S, i64 E,
+                                                u32 (&out)[L]) {
+  constexpr int NL = R28<L>::NL;
+  constexpr int C0 = ProdCut<L>::C0, NWH = ProdCut<L>::NWH;
+  if (E == B3D_EZ || ea == B3D_EZ) {
+    mp::zero(out);
+    return B3D_EZ;
+  }
+  auto term = [&](int j, int32_t (&a)[NL], int32_t (&y)[NL]) -> bool {
+    const int32_t* t = st + j * (NL + 1);
+    if (t[NL] == 0) return false;
+    u32 au[NL];
+    mp::load(au, a28, aps, abase + j);
+    const u32 m = asg[abase + j];
+#pragma unroll
+    for (int k = 0; k < NL; ++k) a[k] = (int32_t)((au[k] ^ m) - m);
+#pragma unroll
+    for (int k = 0; k < NL; ++k) y[k] = t[k];
+    return true;
+  };
+  int32_t acc[2 * NL];
+#pragma unroll
+  for (int c = 0; c < 2 * NL; ++c) acc[c] = 0;
+  for (int j = 0; j < S; ++j) {
+    int32_t a[NL], y[NL];
+    if (term(j, a, y)) mac28_from<C0>(acc, a, y);
+  }
+  u32 U[NWH];
+  const bool sure = upper_sure<L>(acc, U);
+  if (B3D_ANY(!sure)) {
+    if (!sure) {
+      int32_t full[2 * NL];
+#pragma unroll
+      for (int c = 0; c < 2 * NL; ++c) full[c] = 0;
+      for (int j = 0; j < S; ++j) {
+        int32_t a[NL], y[NL];
+        if (term(j, a, y)) mac28(full, a, y);
+      }
+      u32 W[2 * L + 2];
+      pack28(full, W);
+      return mp::norm<L, 2 * L + 2>(W, ea + E, out);
+    }
+  }
+  return mp::norm<L, NWH>(U, ea + E + 28 * C0, out);
+}
+
+// Stage one term for small_dot_staged: X entry idx aligned to the exponent E (magnitude truncated), in 28-bit
+// limbs with its sign, and the flag; a zero entry or one shifted out entirely is a zero term.
+template <int L>
+__device__ __forceinline__ void stage_term(const int32_t* __restrict__ xm, const i64* __restrict__ xe, i64 xps, i64 idx,
+                                           i64 E, int32_t* t) {
+  constexpr int NL = R28<L>::NL;
+  u32 x[L];
+  mp::load(x, xm, xps, idx);
+  const i64 sh = E - xe[idx];
+  if (x[L - 1] == 0u || E == B3D_EZ || sh >= 32 * L) {
+    t[NL] = 0;
+    return;
+  }
+  const u32 sx = mp::to_mag(x);
+  mp::shr(x, (int)sh);
+  int32_t y[NL];
+  to28(x, sx, y);
+#pragma unroll
+  for (int k = 0; k < NL; ++k) t[k] = y[k];
+  t[NL] = 1;
+}
+
 // a two's complement value of L limbs as 28-bit magnitude limbs and a sign mask (the pole step's constant form)
 template <int L>
 __device__ __forceinline__ u32 split28(u32 (&a)[L], u32 (&a28)[R28<L>::NL]) {
