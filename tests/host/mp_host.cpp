@@ -136,6 +136,34 @@ template <int L> static int64_t tri_tl(int TL, const int32_t* vm, const int64_t*
   return tri_t<L, 8>(vm, ve, vps, vbase, vstride, K, e, tm, tsg, tps, tb, texp, r);
 }
 
+// small_dot_staged on host arrays: the S terms staged by stage_term into a local buffer first
+template <int L> static int64_t dot_staged_t(const int32_t* am, int64_t aps, int64_t abase, int64_t ea,
+                                             const int32_t* xm, const int64_t* xe, int64_t xps, int64_t xbase,
+                                             int64_t xstride, int S, uint32_t* r) {
+  constexpr int NL = mp::R28<L>::NL;
+  // the constant row as 28-bit magnitude limbs + sign masks, as b3d_mpf_to28 stores it
+  int32_t a28[NL * 16];
+  uint32_t sg[16];
+  for (int j = 0; j < S; ++j) {
+    u32 a[L];
+    for (int k = 0; k < L; ++k) a[k] = (u32)am[k * aps + abase + j];
+    u32 t[NL];
+    sg[j] = mp::split28<L>(a, t);
+    for (int k = 0; k < NL; ++k) a28[k * S + j] = (int32_t)t[k];
+  }
+  int64_t E = B3D_EZ;
+  for (int j = 0; j < S; ++j) {
+    const int64_t idx = xbase + j * xstride;
+    if (xm[(L - 1) * xps + idx] != 0) E = xe[idx] > E ? xe[idx] : E;
+  }
+  int32_t st[16 * (NL + 1)];
+  for (int j = 0; j < S; ++j) mp::stage_term<L>(xm, xe, xps, xbase + j * xstride, E, st + j * (NL + 1));
+  u32 R[L];
+  const int64_t e = mp::small_dot_staged<L>(a28, sg, S, 0, ea, st, S, E, R);
+  copy_out(r, R);
+  return e;
+}
+
 extern "C" {
 int64_t hc_add(int L, const uint32_t* a, int64_t ea, const uint32_t* b, int64_t eb, uint32_t* r) {
   DISP(L, add_t<LL>(a, ea, b, eb, r))
@@ -163,5 +191,9 @@ int64_t hc_mom(int L, const int32_t* sm, const int64_t* se, int64_t sps, int W, 
 int64_t hc_tri(int L, int TL, const int32_t* vm, const int64_t* ve, int64_t vps, int64_t vbase, int64_t vstride, int K,
                int e, const int32_t* tm, const uint32_t* tsg, int64_t tps, int64_t tb, int64_t texp, uint32_t* r) {
   DISP(L, tri_tl<LL>(TL, vm, ve, vps, vbase, vstride, K, e, tm, tsg, tps, tb, texp, r))
+}
+int64_t hc_dot_staged(int L, const int32_t* am, int64_t aps, int64_t abase, int64_t ea, const int32_t* xm,
+                      const int64_t* xe, int64_t xps, int64_t xbase, int64_t xstride, int S, uint32_t* r) {
+  DISP(L, dot_staged_t<LL>(am, aps, abase, ea, xm, xe, xps, xbase, xstride, S, r))
 }
 }

@@ -30,7 +30,7 @@ def lib():
     os.replace(tmp, SO)
     lb = ctypes.CDLL(SO)
     for f in (lb.hc_add, lb.hc_mul, lb.hc_norm, lb.hc_div, lb.hc_shift, lb.hc_cauchy, lb.hc_dot, lb.hc_mom,
-              lb.hc_tri):
+              lb.hc_tri, lb.hc_dot_staged):
         f.restype = I64
     return lb
 
@@ -216,7 +216,13 @@ def test_small_dot_cancellation(lib, L):
         r = np.zeros(L, np.uint32)
         e = lib.hc_dot(L, ptr(A), I64(S), I64(0), I64(ea), ptr(X), ptr(XE), I64(S), I64(0), I64(1), S, ptr(r))
         xm, xe = R.align(xs)
-        assert got(r, e) == R.gemm_aligned(am_, ea, xm, xe, L), (trial, kind)
+        want = R.gemm_aligned(am_, ea, xm, xe, L)
+        assert got(r, e) == want, (trial, kind)
+        # the LDS-staged form of the kernels (terms aligned and converted first, constant's sign in its limbs)
+        r2 = np.zeros(L, np.uint32)
+        e2 = lib.hc_dot_staged(L, ptr(A), I64(S), I64(0), I64(ea), ptr(X), ptr(XE), I64(S), I64(0), I64(1), S,
+                               ptr(r2))
+        assert got(r2, e2) == want, ("staged", trial, kind)
 
 
 @pytest.mark.parametrize("L", [4, 10, 34])
