@@ -210,7 +210,7 @@ PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_cauchy_nmax6_1024bit_pmc.txt")
 
 
 def pmc_traffic(workload):
-    """HBM bytes per k_cauchy launch from the committed rocprofv3 --pmc summary of this workload
+    """HBM bytes per Cauchy-step launch from the committed rocprofv3 --pmc summary of this workload
     (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by its reading for a plain copy of known size),
     or None when there is no summary for this workload"""
     try:
@@ -224,8 +224,8 @@ def pmc_traffic(workload):
 
 
 def roofline(timer, workload):
-    """The Cauchy step (k_cauchy_groups, one launch per level), the kernel with the largest share of
-    device time: algorithmic bytes per launch (gathered
+    """The Cauchy step (one launch per level: k_cauchy_pairs on large levels, k_cauchy_groups on small
+    ones), the kernel with the largest share of device time: algorithmic bytes per launch (gathered
     residues, h_inf entries and H entries, each counted once) over its average launch time, both from the
     timed region (HIP events recorded around every launch on the launch stream)"""
     import torch
@@ -235,7 +235,7 @@ def roofline(timer, workload):
     avg_s = sum(ms) / len(ms) * 1e-3
     per_launch = nbytes / len(timer)
     achieved = per_launch / avg_s / 1e9
-    return {"kernel": "k_cauchy_groups<34>", "bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
+    return {"kernel": "Cauchy step: k_cauchy_pairs<34> / k_cauchy_groups<34>", "bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
             "frac": achieved / 8000.0, "traffic": pmc_traffic(workload), "launches": len(timer),
             "avg_launch_ms": avg_s * 1e3, "algorithmic_bytes_per_launch": per_launch}
 

@@ -5,7 +5,8 @@
 //              gathers the prime-pole residues straight from the slot buffer (no staging copy), aligns
 //              them per column, divides by the one-limb integer D' - D_i, sums exactly, rounds once and
 //              adds the h_inf entry.  k_cauchy_groups with a wave table (a group's rows over consecutive
-//              lanes, the group's data in scalar registers); without one, k_cauchy_rows when the rows,
+//              lanes, the group's data in scalar registers) or k_cauchy_pairs (the same with the rows in pairs,
+//              one lane both rows of a pair); without one, k_cauchy_rows when the rows,
 //              padded to whole waves, keep at least 3/4 of the lanes busy, k_cauchy otherwise;
 //   k_pole_r   Y[t, b0, a, k] = sum_{b < SR} Ralign[R_t][b0][b] * H[hrow_t][a, b][k]
 //   k_pole_l   res[slot_t][a0, b0][k] = sum_{a < SL} QLalign[qL_t][a0][a] * Y[t, b0, a, k]
@@ -78,6 +79,43 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L <= 34 ? 5
   const i64 t = h * N + n;
   mp::store(r, om, ops, t);
   oe[t] = er;
+}
+
+// Group-uniform form, two rows per lane: the rows of one group are taken in pairs, and the pairs laid out column by column over consecutive lanes, a wave at a time, padded
+// only at the end of the group: a wave may span several pairs of its group, so only the rows' positions D'
+// (VGPRs) vary across its lanes, while the group's column list, divisor positions and plane bases are
+// scalars.  A lane computes both rows of its pair at its column (mp::cauchy_entry2): each term is loaded and
+// put on the grid once for the two, only the division and the sum are per row.  Wave w covers pair entries
+// woff[w] + lane of the group whose rows start at wrow[w] (wcnt[w] rows, ceil(wcnt[w] / 2) pairs, N columns
+// each).  The two accumulators take the kernel to 168 VGPRs up to L = 34, three waves per SIMD (asked for:
+// at four the compiler spills): it does a third less work per entry than k_cauchy_groups (a term's load and
+// shift are about a third of its cost, the division by D - D_i the rest) with fewer waves in flight, so it
+// pays on levels with many waves to run (15% on the heaviest nmax = 6 group) and not on small ones.
+template <int L>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L <= 34 ? 3 : 1))) void k_cauchy_pairs(
+    const int32_t* __restrict__ rm, const i64* __restrict__ re, i64 rps, i64 N, const int32_t* __restrict__ hm,
+    const i64* __restrict__ he, i64 hps, const int64_t* __restrict__ ghrow, const int32_t* __restrict__ rg,
+    const int32_t* __restrict__ rd, const int64_t* __restrict__ col, const int32_t* __restrict__ cold,
+    const int32_t* __restrict__ ncol, int Kc, int32_t* __restrict__ om, i64* __restrict__ oe, i64 ops,
+    const int32_t* __restrict__ wrow, const int32_t* __restrict__ wcnt, const int32_t* __restrict__ woff, i64 nw) {
+  const unsigned lb = b3d_xcd_block(blockIdx.x, gridDim.x);   // a group's waves stay on one XCD's L2
+  const i64 w = (i64)lb * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (w >= nw) return;
+  const int h0 = __builtin_amdgcn_readfirstlane(wrow[w]);
+  const int cnt = __builtin_amdgcn_readfirstlane(wcnt[w]);
+  const i64 o = (i64)__builtin_amdgcn_readfirstlane(woff[w]) + (threadIdx.x & 63);
+  if (o >= (i64)((cnt + 1) / 2) * N) return;
+  const int g = __builtin_amdgcn_readfirstlane(rg[h0]);
+  const i64 p = o / N, n = o % N;
+  const i64 ha = h0 + 2 * p;
+  const bool has2 = 2 * p + 1 < cnt;
+  u32 r[L];
+  i64 e1, e2;
+  mp::cauchy_entry2<L>(rm, re, rps, N, n, col + (i64)g * Kc, cold + (i64)g * Kc, ncol[g], rd[ha],
+                       has2 ? rd[ha + 1] : 0, has2, hm, he, hps, ghrow[g], r, e1, e2, om, ops, ha * N + n,
+                       (ha + 1) * N + n);
+  oe[ha * N + n] = e1;
+  if (has2) oe[(ha + 1) * N + n] = e2;
 }
 
 // General form (rows that would leave many lanes idle if padded to whole waves): every index per lane.
@@ -200,7 +238,7 @@ int b3d_cauchy_level(int L, const int32_t* rm, const int64_t* re, int64_t rps, i
                      const int64_t* he, int64_t hps, const int64_t* ghrow, const int32_t* rg, const int32_t* rd,
                      int64_t nH, const int64_t* col, const int32_t* cold, const int32_t* ncol, int Kc, int32_t* om,
                      int64_t* oe, int64_t ops, const int32_t* wrow, const int32_t* wcnt, const int32_t* woff,
-                     int64_t nw, void* stream) {
+                     int64_t nw, int rows_per_lane, void* stream) {
   if (nH <= 0 || N <= 0) return 0;
   if (b3d::check_planes("b3d_cauchy_level", {rps, hps, ops})) return 2;
   hipStream_t s = (hipStream_t)stream;
@@ -209,7 +247,14 @@ int b3d_cauchy_level(int L, const int32_t* rm, const int64_t* re, int64_t rps, i
       b3d::set_error("b3d_cauchy_level: too many waves");
       return 2;
     }
-    B3D_DISPATCH_L(L, k_cauchy_groups<LL><<<dim3((unsigned)((nw + 3) / 4)), dim3(256), 0, s>>>(rm, re, rps, N, hm, he, hps, ghrow, rg, rd, col, cold, ncol, Kc, om, oe, ops, wrow, wcnt, woff, nw));
+    if (rows_per_lane == 2) {
+      B3D_DISPATCH_L(L, k_cauchy_pairs<LL><<<dim3((unsigned)((nw + 3) / 4)), dim3(256), 0, s>>>(rm, re, rps, N, hm, he, hps, ghrow, rg, rd, col, cold, ncol, Kc, om, oe, ops, wrow, wcnt, woff, nw));
+    } else if (rows_per_lane == 1) {
+      B3D_DISPATCH_L(L, k_cauchy_groups<LL><<<dim3((unsigned)((nw + 3) / 4)), dim3(256), 0, s>>>(rm, re, rps, N, hm, he, hps, ghrow, rg, rd, col, cold, ncol, Kc, om, oe, ops, wrow, wcnt, woff, nw));
+    } else {
+      b3d::set_error("b3d_cauchy_level: rows_per_lane must be 1 or 2");
+      return 2;
+    }
     return b3d::check_launch("b3d_cauchy_level");
   }
   // row-uniform kernel unless padding the rows to whole waves would idle more than a quarter of the lanes

@@ -847,6 +847,86 @@ __device__ __forceinline__ i64 cauchy_entry(const int32_t* __restrict__ rm, cons
   return norm<L, L + 2>(acc, G, r);
 }
 
+// Two rows of one group at column n (positions D1, D2; has2 false: only the first): cauchy_entry<L, true> for
+// each, sharing what depends on the column alone (the grid G, every term's load and shift, the h_inf part);
+// only the division by D - D_i and the sum are per row.  Same bits as two cauchy_entry calls.
+template <int L>
+__device__ __forceinline__ void cauchy_entry2(const int32_t* __restrict__ rm, const i64* __restrict__ re, i64 rps,
+                                              i64 N, i64 n, const int64_t* __restrict__ colg,
+                                              const int32_t* __restrict__ coldg, int nc, int D1, int D2, bool has2,
+                                              const int32_t* __restrict__ hm, const i64* __restrict__ he, i64 hps,
+                                              i64 hrow, u32 (&r)[L], i64& e1, i64& e2, int32_t* __restrict__ om,
+                                              i64 ops, i64 t1, i64 t2) {
+  i64 E = B3D_EZ;
+#pragma unroll 4
+  for (int i = 0; i < nc; ++i) {
+    const i64 ei = re[colg[i] * N + n];
+    E = ei > E ? ei : E;
+  }
+  const i64 hb = hrow * N;
+  const bool hz = load_limb(hm + hb, hps, L - 1, n) == 0u;
+  const i64 eh = he[hb + n];
+  if (E == B3D_EZ && hz) {
+    zero(r);
+    store(r, om, ops, t1);
+    if (has2) store(r, om, ops, t2);
+    e1 = e2 = B3D_EZ;
+    return;
+  }
+  const i64 G = E == B3D_EZ ? eh - 32 : (hz ? E - 31 : ((E - 31) > (eh - 32) ? (E - 31) : (eh - 32)));
+  u32 acc1[L + 2], acc2[L + 2];
+  zero(acc1);
+  zero(acc2);
+  if (E != B3D_EZ) {
+    i64 cn = colg[0];
+    int dn = coldg[0];
+    for (int i = 0; i < nc; ++i) {
+      const i64 c = cn;
+      const int d1 = D1 - dn, d2 = has2 ? D2 - dn : 0;
+      if (i + 1 < nc) {
+        cn = colg[i + 1];
+        dn = coldg[i + 1];
+      }
+      if (d1 == 0 && d2 == 0) continue;
+      u32 X[L + 1];
+      X[0] = 0u;
+      load_desc<L>(X, rm + c * N, rps, n);
+      if (X[L] == 0u) continue;
+      const i64 sh = G + 31 - re[c * N + n];
+      if (sh >= 32 * (L + 1)) continue;
+      const u32 fill = 0u - (X[L] >> 31);
+      sar(X, (int)sh, fill);
+      if (d1 != 0) {
+        u32 Q[L + 1];
+#pragma unroll
+        for (int k = 0; k <= L; ++k) Q[k] = X[k];
+        div_floor(Q, (u32)(d1 < 0 ? -d1 : d1));
+        cadd(acc1, Q, 0u - (u32)(d1 < 0), fill);
+      }
+      if (d2 != 0) {
+        div_floor(X, (u32)(d2 < 0 ? -d2 : d2));
+        cadd(acc2, X, 0u - (u32)(d2 < 0), fill);
+      }
+    }
+  }
+  if (!hz) {
+    u32 y[L + 1];
+    y[0] = 0u;
+    load_desc<L>(y, hm + hb, hps, n);
+    const u32 sh_ = to_mag(y);
+    const i64 s = 32 - (eh - G);
+    sar(y, s > 32 * (L + 1) ? 32 * (L + 1) : (int)s, 0u);
+    cadd(acc1, y, sh_);
+    cadd(acc2, y, sh_);
+  }
+  e1 = norm<L, L + 2>(acc1, G, r);
+  store(r, om, ops, t1);
+  if (has2) {
+    e2 = norm<L, L + 2>(acc2, G, r);
+    store(r, om, ops, t2);
+  }
+}
+
 // One windowed moment: sum_w y_w n_w^k for the staged entries y_w = stage[w, row, c] (w < W), n_w = n0 + w,
 // exact n^k from NK (KL limb planes, entry n * R1 + k, bit lengths NKb).  E = max_w (e_w + NB + bitlen n_w^k)
 // bounds the top bit; every product is put on the grid 2^T, T = E - 32 (L + 1) (truncated), summed exactly

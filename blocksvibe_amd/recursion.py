@@ -112,6 +112,11 @@ class Recursion:
     mma = os.environ.get("BLOCKSVIBE_POLE_MMA", "0") == "1"
     # the Cauchy step's group-uniform kernel (per-level wave tables); False: the library's kernels without them
     group_waves = True
+    # two rows per lane (k_cauchy_pairs) on levels with at least this many waves of pairs (one wave for each
+    # of the 1024 SIMDs at the kernel's three waves per SIMD), one row per lane below it, where the level
+    # is too small to fill the chip and the one-row kernel's five waves per SIMD win; measured on the nmax = 6
+    # 1024-bit workload: 1536 .. 6144 give the same time, 0 (pairs everywhere) and never are 5% slower
+    pair_min_waves = 3072
 
     def _constants(self):
         plan, Sm, L = self.plan, self.plan.Smax, self.L
@@ -158,10 +163,13 @@ class Recursion:
         self._waves = {}
 
     def wave_tables(self, N):
-        """per level, the Cauchy step's wave table for N columns per row (cached per N): each group's rows
-        (contiguous in the level's row order) laid out column-major over consecutive lanes, 64 lanes a wave,
-        padded at the end of each group -> (first row of the group, rows of the group, first entry) per wave"""
-        if N not in self._waves:
+        """per level, the Cauchy step's wave table for N columns per row (cached per N and pair_min_waves):
+        each group's rows (contiguous in the level's row order) taken one or two at a time (pairs 2p, 2p + 1;
+        an odd count ends with one row), laid out column-major over consecutive lanes, 64 lanes a wave,
+        padded at the end of each group -> (first row of the group, rows of the group, first unit, waves,
+        rows per lane): two when the level has at least pair_min_waves waves of pairs"""
+        key = (N, self.pair_min_waves)
+        if key not in self._waves:
             d, tabs = mpf.dev(), []
             for lev in self.plan.levels:
                 if lev is None:
@@ -171,15 +179,16 @@ class Recursion:
                 starts = np.flatnonzero(np.r_[True, rg[1:] != rg[:-1]])
                 assert len(starts) == len(np.unique(rg)), "a level's rows must be grouped"
                 cnts = np.diff(np.r_[starts, len(rg)])
-                nwg = (cnts * N + 63) // 64
+                rpl = 2 if int((((cnts + 1) // 2 * N + 63) // 64).sum()) >= self.pair_min_waves else 1
+                nwg = ((cnts + rpl - 1) // rpl * N + 63) // 64
                 wrow = np.repeat(starts, nwg).astype(np.int32)
                 wcnt = np.repeat(cnts, nwg).astype(np.int32)
                 woff = ((np.arange(int(nwg.sum())) - np.repeat(np.cumsum(nwg) - nwg, nwg)) * 64).astype(np.int32)
                 assert cnts.max() * N < 1 << 31
                 tabs.append((torch.from_numpy(wrow).to(d), torch.from_numpy(wcnt).to(d), torch.from_numpy(woff).to(d),
-                             len(wrow)))
-            self._waves[N] = tabs
-        return self._waves[N]
+                             len(wrow), rpl))
+            self._waves[key] = tabs
+        return self._waves[key]
 
     def run(self, h_inf, hook=None, prof=None, res=None, timer=None):
         """h_inf: MPF (2, order + 1, Jh, SL SR, B) (structure entry (a, b) at a SR + b).  Returns the residue
@@ -201,7 +210,7 @@ class Recursion:
         Y = mpf.zeros((self.maxT, SR, SL, B), L)
         st = mpf.stream_ptr()
         P = lambda t: t.data_ptr()
-        waves = self.wave_tables(N) if self.group_waves else [(None, None, None, 0)] * len(self.levels)
+        waves = self.wave_tables(N) if self.group_waves else [(None, None, None, 0, 1)] * len(self.levels)
         for lv, wt in zip(self.levels, waves):
             if lv is None:
                 continue
@@ -212,7 +221,7 @@ class Recursion:
             _lib.call("b3d_cauchy_level", L, P(res.m), P(res.e), res.numel(), N, P(h.m), P(h.e), h.numel(),
                       P(lv["ghrow"]), P(lv["rg"]), P(lv["rd"]), lv["nH"], P(lv["col"]), P(lv["cold"]),
                       P(lv["ncol"]), lv["Kc"], P(H.m), P(H.e), H.numel(), *(P(x) if x is not None else 0 for x in wt[:3]),
-                      wt[3], st)
+                      wt[3], wt[4], st)
             if timer is not None:
                 ev1 = torch.cuda.Event(enable_timing=True)
                 ev1.record()

@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-#define B3D_ABI_VERSION 11
+#define B3D_ABI_VERSION 12
 
 /* Last error message of the calling thread (empty if none). */
 const char* b3d_last_error(void);
@@ -77,15 +77,19 @@ int b3d_mpf_align_rows(int L, const int32_t* m, const int64_t* e, int64_t ps, in
  * stride hps, N columns per row); groups g < nb: ghrow[g] = h_inf row, col[g*Kc + i] physical slots of
  * the prime poles (i < ncol[g]), cold[g*Kc + i] their doubled positions D_i; H rows h < nH: rg[h] group,
  * rd[h] doubled D'.  Output H (nH, N) at om/oe (plane stride ops).  Requires |D' - D_i| < 2^15.
- * Optional wave table (null: none): the rows of a group contiguous, wave w < nw covers entries
- * woff[w] .. woff[w] + 63 (column-major within the group: entry o is row wrow[w] + o / N, column o % N) of
- * the group whose wcnt[w] rows start at row wrow[w]; every entry of every group in exactly one wave.  With
- * it the group's data is wave-uniform and rows are padded only at the end of their group. */
+ * Optional wave table (null: none), for rows_per_lane 1 or 2: the rows of a group contiguous and taken
+ * rows_per_lane at a time (rows_per_lane 2: rows 2p, 2p + 1 of the group, the last pair of an odd count
+ * has one row); wave w < nw covers units woff[w] .. woff[w] + 63 (unit o is row / pair o / N of the group,
+ * column o % N) of the group whose wcnt[w] rows start at row wrow[w]; every unit of every group in exactly
+ * one wave, ceil(wcnt / rows_per_lane) N of them per group.  With it the group's data is wave-uniform and
+ * rows are padded only at the end of their group; with rows_per_lane 2 a lane computes both rows of its
+ * pair at its column (one load and shift of every term for both: less work per entry, fewer waves in
+ * flight; the same bits).  rows_per_lane is ignored without a table. */
 int b3d_cauchy_level(int L, const int32_t* rm, const int64_t* re, int64_t rps, int64_t N, const int32_t* hm,
                      const int64_t* he, int64_t hps, const int64_t* ghrow, const int32_t* rg, const int32_t* rd,
                      int64_t nH, const int64_t* col, const int32_t* cold, const int32_t* ncol, int Kc, int32_t* om,
                      int64_t* oe, int64_t ops, const int32_t* wrow, const int32_t* wcnt, const int32_t* woff,
-                     int64_t nw, void* stream);
+                     int64_t nw, int rows_per_lane, void* stream);
 
 /* Pole step for T tasks: Y = Ralign[tR[t]] . H[thrw[t]]^T, then res[tslot[t]] = QLalign[tQL[t]] . Y^T, over
  * residue blocks of SL x SR structure entries (left a < SL, right b < SR; entry (a, b) of a residue at

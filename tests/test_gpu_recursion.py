@@ -127,8 +127,9 @@ def test_recursion_bits_and_exact(case):
 @pytest.mark.parametrize("case", [CASES[1], CASES[2], CASES[6], CASES[3]],
                          ids=lambda c: f"{c['tj12']}-{c['tj43']}-B{c['B']}")
 def test_cauchy_kernels_agree(case):
-    """the group-uniform Cauchy kernel (wave tables) against the kernels used without them (rows padded to
-    whole waves at 63 and 64 columns, every index per lane at 75 and 4 columns), bit for bit"""
+    """the group-uniform Cauchy kernels (wave tables; one row per lane and two) against the kernels used
+    without them (rows padded to whole waves at 63 and 64 columns, every index per lane at 75 and 4
+    columns), bit for bit"""
     from blocksvibe_amd.plan import Plan
     from blocksvibe_amd.recursion import Recursion
     c = case
@@ -138,9 +139,10 @@ def test_cauchy_kernels_agree(case):
     h = h_tensor(H, plan, L)
     print(f"cauchy kernels: {plan.S2 * B} columns per row", flush=True)
     outs = []
-    for waves in (True, False):
+    for waves, pmin in ((True, 0), (True, 1 << 62), (False, 0)):
         rec = Recursion(plan, M, L)
-        rec.group_waves = waves
+        rec.group_waves, rec.pair_min_waves = waves, pmin
         outs.append(rec.run(h))
-    a, b = outs
-    assert torch.equal(a.e, b.e) and torch.equal(a.m, b.m)
+    a = outs[0]
+    for b in outs[1:]:
+        assert torch.equal(a.e, b.e) and torch.equal(a.m, b.m)
