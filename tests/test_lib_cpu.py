@@ -29,3 +29,14 @@ def test_limb_rule():
     # reference cli.py:270-274: 256 -> 10, 512 -> 18, 768 -> 26, 1024 -> 34, 1280 -> 42, 1536 -> 50
     assert [limbs_for_precision(p) for p in (256, 512, 768, 1024, 1280, 1536)] == [10, 18, 26, 34, 42, 50]
     assert limbs_for_precision(384) == 14 and limbs_for_precision(200) == 10
+
+
+def test_cauchy_rows_per_lane_checked():
+    """b3d_cauchy_level rejects a wave table for other than one or two rows per lane before it launches
+    anything (the pointers below are never dereferenced)"""
+    import numpy as np
+    from blocksvibe_amd import _lib
+    buf = np.zeros(64, np.int64)
+    p = buf.ctypes.data
+    with pytest.raises(_lib.B3DError, match="rows_per_lane"):
+        _lib.call("b3d_cauchy_level", 34, p, p, 1, 1, p, p, 1, p, p, p, 1, p, p, p, 1, p, p, 1, p, p, p, 1, 3, 0)
