@@ -17,6 +17,8 @@
 #include "mp.hpp"
 #include "../../include/b3dhip.h"
 
+#include <mutex>
+
 typedef int v4i __attribute__((ext_vector_type(4)));
 
 static inline unsigned grid_for(int64_t n, int th) {
@@ -436,6 +438,84 @@ int b3d_rns_gemm(int L, int nmod, const uint16_t* primes, const int8_t* w8, cons
   }
   (void)bplane;
   return 0;
+}
+
+}  // extern "C"
+
+// ---- the allocation-free form (b3d_rns_gemm_noalloc) ----
+// Its scratch: the column exponents EB, then the workspace of one column chunk.  A device global, so each
+// GPU the library runs on holds its own copy (at its own address) from the module's load until the process
+// ends.  Host code never uses this name as an address: the per-device address comes from
+// hipGetSymbolAddress.
+static __device__ __align__(256) char scratch[B3D_RNS_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch[kMaxDevices] = {};
+
+// the scratch's device address on the current device (looked up once per device), or null with the error set
+char* device_scratch() {
+  int dev = -1;
+  hipError_t st = hipGetDevice(&dev);
+  if (st != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    b3d::set_error(std::string("b3d_rns_gemm_noalloc: no usable current device (") + hipGetErrorString(st) + ")");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    st = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (st != hipSuccess || !p) {
+      b3d::set_error(std::string("b3d_rns_gemm_noalloc: hipGetSymbolAddress(scratch) failed: ") +
+                     hipGetErrorString(st));
+      return nullptr;
+    }
+    g_scratch[dev] = (char*)p;
+  }
+  return g_scratch[dev];
+}
+}  // namespace
+
+extern "C" {
+
+int b3d_rns_gemm_noalloc(int L, int nmod, const uint16_t* primes, const int8_t* w8, const int32_t* wcorr,
+                         const uint16_t* inv, const double* rp, const int8_t* c8, int CK, int64_t nb, int64_t M,
+                         int64_t N, int64_t K, int a_shared, const int32_t* Am, const int64_t* Ae, int64_t aps,
+                         int64_t asb, int64_t asi, int64_t ask, const int32_t* Bm, const int64_t* Be, int64_t bps,
+                         int64_t bsb, int64_t bsk, int64_t bsj, int32_t* Cm, int64_t* Ce, int64_t cps, int64_t csb,
+                         int64_t csi, int64_t csj, int tile_wm, void* stream) {
+  if (nb <= 0 || M <= 0 || N <= 0) return 0;
+  if (K <= 0 || K > (1 << 16) || nmod <= 0 || nmod > 255 || nb > 65535 || M > (int64_t(1) << 30) ||
+      N > (int64_t(1) << 30)) {
+    b3d::set_error("b3d_rns_gemm_noalloc: need 0 < K <= 2^16, 0 < nmod <= 255, nb < 65536, M, N <= 2^30");
+    return 2;
+  }
+  const int64_t cap = B3D_RNS_NOALLOC_SCRATCH_BYTES;
+  const int64_t nbA = a_shared ? 1 : nb;
+  const int64_t eb_bytes = (nb * N * 8 + 255) / 256 * 256;
+  const int64_t room = cap - eb_bytes;
+  // the widest column chunk whose workspace fits beside EB (the workspace grows by `per` for every 64 columns)
+  const int64_t fixed = b3d_rns_workspace_bytes(nmod, nbA, nb, M, 0, K);
+  const int64_t per = b3d_rns_workspace_bytes(nmod, nbA, nb, M, 64, K) - fixed;
+  int64_t Nc = N;
+  if (room <= 0 || b3d_rns_workspace_bytes(nmod, nbA, nb, M, N, K) > room) {
+    Nc = room > fixed ? (room - fixed) / per * 64 : 0;
+    if (Nc > N) Nc = N;
+  }
+  if (Nc < 1 || b3d_rns_workspace_bytes(nmod, nbA, nb, M, Nc, K) > room) {
+    const int64_t need = eb_bytes + b3d_rns_workspace_bytes(nmod, nbA, nb, M, N < 64 ? N : 64, K);
+    b3d::set_error("b3d_rns_gemm_noalloc: this product needs at least " + std::to_string(need) +
+                   " bytes of scratch (column exponents and one chunk of " + std::to_string(N < 64 ? N : 64) +
+                   " columns) but the fixed scratch holds " + std::to_string(cap) +
+                   " bytes; use b3d_rns_gemm with a workspace");
+    return 2;
+  }
+  char* base = device_scratch();
+  if (!base) return 1;
+  return b3d_rns_gemm(L, nmod, primes, w8, wcorr, inv, rp, c8, CK, nb, M, N, K, a_shared, Am, Ae, aps, asb, asi, ask,
+                      Bm, Be, bps, bsb, bsk, bsj, Cm, Ce, cps, csb, csi, csj, Nc, tile_wm, (int64_t*)base,
+                      base + eb_bytes, room, stream);
 }
 
 }  // extern "C"

@@ -241,6 +241,23 @@ def gemm_strided(L, nb, M, N, K, A, a_strides, a_shared, B, b_strides, C, c_stri
     return C
 
 
+def gemm_strided_noalloc(L, nb, M, N, K, A, a_strides, a_shared, B, b_strides, C, c_strides, tile=1):
+    """gemm_strided through b3d_rns_gemm_noalloc: no workspace and no allocation by the library (its fixed
+    per-GPU scratch; B3DError if the product does not fit in it).  The residue tables are built on the first
+    use of an (L, nmod) pair, so a caller that must not allocate at all calls once beforehand."""
+    if nb == 0 or M == 0 or N == 0:
+        return C
+    nmod = n_moduli(L, K)
+    t = tables(L, nmod)
+    P = lambda x: x.data_ptr()
+    _lib.call("b3d_rns_gemm_noalloc", L, nmod, P(t["primes"]), P(t["w8"]), P(t["wcorr"]), P(t["inv"]), P(t["rp"]),
+              P(t["c8"]), t["CK"], nb, M, N, K, 1 if a_shared else 0,
+              P(A.m), P(A.e), A.m[0].numel(), *a_strides,
+              P(B.m), P(B.e), B.m[0].numel(), *b_strides,
+              P(C.m), P(C.e), C.m[0].numel(), *c_strides, tile, mpf.stream_ptr())
+    return C
+
+
 def gemm(A, B, out=None, budget=None, tile=None):
     """A: MPF (M, K) (shared) or (nb, M, K); B: MPF (K, N) or (nb, K, N) -> MPF (M, N) or (nb, M, N)"""
     A, B = A.contiguous(), B.contiguous()

@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-#define B3D_ABI_VERSION 12
+#define B3D_ABI_VERSION 13
 
 /* Last error message of the calling thread (empty if none). */
 const char* b3d_last_error(void);
@@ -172,6 +172,25 @@ int b3d_rns_gemm(int L, int nmod, const uint16_t* primes, const int8_t* w8, cons
                  const int32_t* Bm, const int64_t* Be, int64_t bps, int64_t bsb, int64_t bsk, int64_t bsj, int32_t* Cm,
                  int64_t* Ce, int64_t cps, int64_t csb, int64_t csi, int64_t csj, int64_t Nc, int tile_wm,
                  int64_t* EB, void* workspace, int64_t ws_bytes, void* stream);
+
+/* Bytes of the fixed device scratch that b3d_rns_gemm_noalloc works in (one copy per GPU, held until the
+ * process ends). */
+#define B3D_RNS_NOALLOC_SCRATCH_BYTES (64ll << 20)
+
+/* b3d_rns_gemm without a workspace argument and without any allocation: the column exponents and the column
+ * chunks' workspace live in a fixed device scratch of B3D_RNS_NOALLOC_SCRATCH_BYTES per GPU, the chunk width
+ * chosen to fit; the same bits as b3d_rns_gemm.  Returns 2 with a message if the column exponents (8 nb N
+ * bytes) and a chunk of min(N, 64) columns do not fit.  The scratch address is looked up once per device on
+ * the first call made with that device current (the stream must belong to the current device).  Every call
+ * on one GPU shares that GPU's scratch: calls on one stream are ordered, calls that overlap on different
+ * streams of one GPU race.  No counterpart in the reference: for callers that cannot allocate device memory
+ * (code under stream capture, tight serving loops, harnesses that forbid hipMalloc). */
+int b3d_rns_gemm_noalloc(int L, int nmod, const uint16_t* primes, const int8_t* w8, const int32_t* wcorr,
+                         const uint16_t* inv, const double* rp, const int8_t* c8, int CK, int64_t nb, int64_t M,
+                         int64_t N, int64_t K, int a_shared, const int32_t* Am, const int64_t* Ae, int64_t aps,
+                         int64_t asb, int64_t asi, int64_t ask, const int32_t* Bm, const int64_t* Be, int64_t bps,
+                         int64_t bsb, int64_t bsk, int64_t bsj, int32_t* Cm, int64_t* Ce, int64_t cps, int64_t csb,
+                         int64_t csi, int64_t csj, int tile_wm, void* stream);
 
 /* ---- output emission (host code; host arrays; no GPU needed) ---- */
 
