@@ -221,10 +221,15 @@ __global__ __launch_bounds__(256) void k_fmt_text(const int32_t* __restrict__ m,
 }
 
 __global__ __launch_bounds__(256) void k_fmt_pack(const uint8_t* __restrict__ tx, const int32_t* __restrict__ len,
-                                                  const uint8_t* __restrict__ last, const int64_t* __restrict__ off,
+                                                  const uint8_t* __restrict__ last, const int32_t* __restrict__ pre,
+                                                  const int32_t* __restrict__ post, const uint8_t* __restrict__ strtab,
+                                                  const int64_t* __restrict__ stroff, const int64_t* __restrict__ off,
                                                   i64 n, uint8_t* __restrict__ out) {
   for (i64 k = blockIdx.x * (i64)blockDim.x + threadIdx.x; k < n; k += (i64)gridDim.x * blockDim.x) {
+    if (last[k] > 1) continue;   // an unused slot: no text
     uint8_t* o = out + off[k];
+    if (pre && pre[k] >= 0)
+      for (i64 c = stroff[pre[k]]; c < stroff[pre[k] + 1]; ++c) *o++ = strtab[c];
     const int l = len[k];
     *o++ = '"';
     for (int c = 0; c < l; ++c) *o++ = tx[(i64)c * n + k];
@@ -233,6 +238,8 @@ __global__ __launch_bounds__(256) void k_fmt_pack(const uint8_t* __restrict__ tx
       *o++ = ',';
       *o++ = '\n';
     }
+    if (post && post[k] >= 0)
+      for (i64 c = stroff[post[k]]; c < stroff[post[k] + 1]; ++c) *o++ = strtab[c];
   }
 }
 
@@ -275,8 +282,22 @@ int b3d_fmt_pack(const uint8_t* text, const int32_t* len, const uint8_t* last, c
                  uint8_t* out, void* stream) {
   if (n <= 0) return 0;
   hipStream_t s = (hipStream_t)stream;
-  k_fmt_pack<<<dim3(grid_for(n, 256)), dim3(256), 0, s>>>(text, len, last, off, n, out);
+  k_fmt_pack<<<dim3(grid_for(n, 256)), dim3(256), 0, s>>>(text, len, last, nullptr, nullptr, nullptr, nullptr, off,
+                                                          n, out);
   return b3d::check_launch("b3d_fmt_pack");
+}
+
+int b3d_fmt_pack_files(const uint8_t* text, const int32_t* len, const uint8_t* last, const int32_t* pre,
+                       const int32_t* post, const uint8_t* strtab, const int64_t* stroff, int64_t nstr,
+                       const int64_t* off, int64_t n, uint8_t* out, void* stream) {
+  if (n <= 0) return 0;
+  if (nstr < 0 || (nstr > 0 && (!strtab || !stroff))) {
+    b3d::set_error("b3d_fmt_pack_files: need a string table (strtab, stroff[nstr + 1]) for nstr > 0");
+    return 2;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  k_fmt_pack<<<dim3(grid_for(n, 256)), dim3(256), 0, s>>>(text, len, last, pre, post, strtab, stroff, off, n, out);
+  return b3d::check_launch("b3d_fmt_pack_files");
 }
 
 }  // extern "C"

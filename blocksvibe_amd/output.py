@@ -159,7 +159,12 @@ class Output:
 
     def apply_per_order(self, nums):
         """nums[idx]: MPF (Jo, Dout_idx, R_idx + 1, 2, S2, nq) for the task's lambda orders idx (all of them for
-        xt, the first for xt_radial).  Returns MPF (Jo, 2, S2, nq, outputs, Dmax)."""
+        xt, the first for xt_radial).  Returns MPF (Jo, 2, S2, nq, outputs, Dmax) (a permuted view of
+        apply_raw's result)."""
+        return self.apply_raw(nums).permute(1, 2, 3, 4, 0, 5)
+
+    def apply_raw(self, nums):
+        """as apply_per_order, in the GEMM's own contiguous layout (outputs, Jo, 2, S2, nq, Dmax)"""
         Dmax = max(x.shape[1] for x in nums)
         rows = []
         for idx, x in enumerate(nums):
@@ -172,5 +177,4 @@ class Output:
             rows.append(X.reshape(R1, -1).take(self._rows[idx]))
         X = mpf.cat(rows, 0)
         Jo, _, _, _, S2, nq = nums[0].shape
-        res = rns.gemm(self.W, X).reshape(len(self.outputs), Jo, 2, S2, nq, Dmax)
-        return res.permute(1, 2, 3, 4, 0, 5)
+        return rns.gemm(self.W, X).reshape(len(self.outputs), Jo, 2, S2, nq, Dmax)

@@ -40,12 +40,21 @@ from .recursion import Recursion
 class Request:
     """one task: operator (tj12, tj43, tjmax, order, Lambda, kept, L, pp, op_id), sign, h_inf per lambda order
     (device MPF (2, order + 1, Jh, S^2, 1)), output coordinates and Delta_1+2.  op_id distinguishes operators
-    with the same shape but different M matrices (Delta_12, Delta_43, external spins)."""
-    __slots__ = ("key", "sign", "hs", "coords", "d12", "spins", "t_arrival", "t_start", "t_done", "result", "done",
-                 "error", "blocks")
+    with the same shape but different M matrices (Delta_12, Delta_43, external spins); M: the operator's
+    matrices as setup.read gives them (None: seeded synthetic ones), used when the operator is first built.
 
-    def __init__(self, key, sign, hs, coords=("xt",), d12="0", spins=None):
+    sink(raw, q, done): optional consumer of the results on the device.  raw maps (coordinate, Delta_1+2) to the
+    batch's write_output result (outputs, Jo, 2, S^2, nq, Dmax) (output.Output.apply_raw), q is this request's
+    column.  It is called on the worker thread with the batch's stream current and must enqueue everything that
+    reads raw on that stream before it returns (the stream is synchronised before raw may be reused); the request
+    completes when it calls done() (from any thread), e.g. once its files are written.  Without a sink, result
+    holds the request's own copy {(coordinate, Delta_1+2): MPF (Jo, 2, S^2, outputs, Dmax)}."""
+    __slots__ = ("key", "sign", "hs", "coords", "d12", "spins", "t_arrival", "t_start", "t_done", "result", "done",
+                 "error", "blocks", "M", "sink")
+
+    def __init__(self, key, sign, hs, coords=("xt",), d12="0", spins=None, M=None, sink=None):
         self.key, self.sign, self.hs, self.coords, self.d12, self.spins = key, sign, hs, tuple(coords), d12, spins
+        self.M, self.sink = M, sink
         self.t_arrival = self.t_start = self.t_done = None
         self.result, self.error = None, None
         self.done = threading.Event()
@@ -63,7 +72,7 @@ class _Operator:
     derivative constants) is shared by every operator of the same shape; only the M-dependent tables are
     built per operator"""
 
-    def __init__(self, key, shapes, seed):
+    def __init__(self, key, shapes, seed, M=None):
         tj12, tj43, tjmax, order, Lam, kept, L, pp, op_id = key
         sk = (tj12, tj43, tjmax, order, pp, L)
         if sk not in shapes:
@@ -71,7 +80,7 @@ class _Operator:
             shapes[sk] = {"plan": plan, "rec": Recursion(plan, None, L, seed=0), "derivs": {}}
         self.shape = shapes[sk]
         self.plan = self.shape["plan"]
-        self.rec = self.shape["rec"].with_matrices(None, seed=seed)
+        self.rec = self.shape["rec"].with_matrices(M, seed=seed)
         self.key = key
 
     def derivs_for(self, p0):
@@ -234,11 +243,15 @@ class Server:
         return sorted(r.t_done - r.t_arrival for r in self.completed)
 
     # ------------------------------------------------------------------------------------------ internals
-    def _operator(self, key):
+    def _operator(self, key, M=None):
         with self.setup_lock:
             if key not in self.ops:
-                self.ops[key] = _Operator(key, self.plans, seed=len(self.ops) + 1)
+                self.ops[key] = _Operator(key, self.plans, seed=len(self.ops) + 1, M=M)
             return self.ops[key]
+
+    def output(self, coord, Lam, p0, d12, L):
+        """the cached write_output map of (coordinate, Lambda, parity, Delta_1+2, L)"""
+        return self._output(coord, Lam, p0, d12, L)
 
     def _output(self, coord, Lam, p0, d12, L):
         k = (coord, Lam, p0, d12, L)
@@ -259,7 +272,7 @@ class Server:
                     self.lock.release()
                     try:
                         with self.gate.eager():
-                            self._operator(head.key)
+                            self._operator(head.key, head.M)
                             torch.cuda.current_stream().synchronize()
                     finally:
                         self.lock.acquire()
@@ -291,22 +304,35 @@ class Server:
                 batch, cost = self._take_batch()
             if batch is None:
                 return
+            deferred = ()
             try:
                 with torch.cuda.stream(stream):
-                    self._run(batch)                # returns with the stream synchronised
+                    deferred = self._run(batch)     # returns with the stream synchronised
                 t = time.time()
                 for r in batch:
-                    r.t_done = t
+                    if r not in deferred:
+                        r.t_done = t
             except Exception as e:          # surface to the callers
+                deferred = ()
                 for r in batch:
                     r.error = e
                     r.t_done = time.time()
+            now = [r for r in batch if r not in deferred]
             with self.lock:
                 self.in_use -= cost
-                self.completed.extend(batch)
+                self.completed.extend(now)
                 self.lock.notify_all()
-            for r in batch:
+            for r in now:
                 r.done.set()
+
+    def _complete(self, r, error=None):
+        """a sink's done(): the request's results are consumed"""
+        r.t_done = time.time()
+        if error is not None:
+            r.error = error
+        with self.lock:
+            self.completed.append(r)
+        r.done.set()
 
     def _pipeline(self, op, key, sign, coords, hs):
         """device work of one batch: per lambda order a recursion over all right-hand sides with the moment
@@ -321,12 +347,12 @@ class Server:
             dn.start(B)
             op.rec.run(h, hook=dn.accumulate)
             nums.append(dn.finish(h))
-        return {(c, d): self._output(c, Lam, p0, d, L).apply_per_order(nums) for c, d in coords}
+        return {(c, d): self._output(c, Lam, p0, d, L).apply_raw(nums) for c, d in coords}
 
     def _run(self, batch):
         t0 = time.time()
         head = batch[0]
-        op = self._operator(head.key)
+        op = self._operator(head.key, head.M)
         for r in batch:
             r.t_start = t0
         coords = tuple(sorted({(c, r.d12) for r in batch for c in r.coords}))
@@ -364,18 +390,27 @@ class Server:
                 with graph.lock:
                     outs = graph.replay(hs)
                     self.replays += 1
-                    self._deliver(batch, outs, copy=True)
+                    deferred = self._deliver(batch, outs)
                     torch.cuda.current_stream().synchronize()
             else:
-                self._deliver(batch, fn(hs), copy=False)
+                deferred = self._deliver(batch, fn(hs))
                 torch.cuda.current_stream().synchronize()
+        return deferred
 
-    @staticmethod
-    def _deliver(batch, outs, copy):
+    def _deliver(self, batch, outs):
+        """each request's results: handed to its sink (on the device), or copied out of the batch's outputs
+        (which a graph replay reuses).  Returns the requests whose sinks complete them later."""
+        deferred = []
         for q, r in enumerate(batch):
+            if r.sink is not None:
+                r.sink(outs, q, lambda err=None, r=r: self._complete(r, err))
+                deferred.append(r)
+                continue
             res = {}
             for k, v in outs.items():
                 if k[0] in r.coords and k[1] == r.d12:
-                    m, e = v.m[:, :, :, :, q], v.e[:, :, :, q]
-                    res[k] = mpf.MPF(m.clone() if copy else m.contiguous(), e.clone() if copy else e.contiguous())
+                    m = v.m[:, :, :, :, :, q].permute(0, 2, 3, 4, 1, 5).contiguous()   # (L, Jo, 2, S2, outputs, D)
+                    e = v.e[:, :, :, :, q].permute(1, 2, 3, 0, 4).contiguous()
+                    res[k] = mpf.MPF(m, e)
             r.result = res
+        return deferred

@@ -231,6 +231,16 @@ int b3d_fmt_text(int L, const int32_t* m, const int64_t* e, int64_t ps, const in
 int b3d_fmt_pack(const uint8_t* text, const int32_t* len, const uint8_t* last, const int64_t* off, int64_t n,
                  uint8_t* out, void* stream);
 
+/* Whole files at once: b3d_fmt_pack's pieces, each element k preceded by string pre[k] and followed by string
+ * post[k] of the table (string i = strtab[stroff[i] .. stroff[i + 1]); -1: none), so that the texts between the
+ * polynomials of a spin file (its header and the JSON brackets) are laid out with them and each file is one
+ * contiguous range of out.  off[k]: offset of element k's piece (pre + '"' text '"' + ",\n" unless last[k] +
+ * post); last[k] = 2 marks an unused slot, which writes nothing.  Ids must be < nstr.  No counterpart in the reference (its writer joins Python strings per
+ * coefficient, reference blocksvibe/b3djson.py:150-207). */
+int b3d_fmt_pack_files(const uint8_t* text, const int32_t* len, const uint8_t* last, const int32_t* pre,
+                       const int32_t* post, const uint8_t* strtab, const int64_t* stroff, int64_t nstr,
+                       const int64_t* off, int64_t n, uint8_t* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
