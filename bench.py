@@ -38,7 +38,7 @@ NQ_PER_SIGN = 11
 BINS = 10
 PP = ((0, 0), (1, 1))
 WORKLOAD = ("nmax=6 spin-2 externals: 25 operator groups x 22 structures x 6 lambda orders, order 60, kept 14, "
-            "1024 bits, xt; 17,050 spin files per pass")
+            "1024 bits, xt")
 METRIC = "blocks/s at nmax=6, 1024-bit vs plain GPU & CPU blocks_3d; p50/p99 latency"
 
 
