@@ -178,3 +178,32 @@ class Output:
         X = mpf.cat(rows, 0)
         Jo, _, _, _, S2, nq = nums[0].shape
         return rns.gemm(self.W, X).reshape(len(self.outputs), Jo, 2, S2, nq, Dmax)
+
+
+def cached(coordinate, Lam, parity, delta_1_plus_2, L, cache_dir=None):
+    """Output whose map is kept on disk under cache_dir (its host construction takes seconds to minutes at
+    large Lambda), keyed by everything the map depends on (reference blocksvibe/cli.py:281-309 keeps the same
+    cache); without cache_dir, or if the file cannot be read or written, the map is built in memory"""
+    import hashlib
+    import os
+    if not cache_dir:
+        return Output(coordinate, Lam, parity, delta_1_plus_2, L)
+    tag = hashlib.sha1(f"{coordinate}|{Lam}|{parity}|{delta_1_plus_2}|{L}|map2".encode()).hexdigest()[:20]
+    path = os.path.join(cache_dir, f"wmap_{coordinate}_L{Lam}_p{parity}_{tag}.pt")
+    try:
+        st = torch.load(path, weights_only=True)
+        W = mpf.MPF(st["m"].to(mpf.dev()), st["e"].to(mpf.dev()))
+        return Output(coordinate, Lam, parity, delta_1_plus_2, L, W=W,
+                      inputs=[tuple(x) for x in st["inputs"].tolist()], outputs=[tuple(x) for x in st["outputs"].tolist()])
+    except (OSError, KeyError, RuntimeError):
+        pass
+    o = Output(coordinate, Lam, parity, delta_1_plus_2, L)
+    try:
+        os.makedirs(cache_dir, exist_ok=True)
+        tmp = f"{path}.{os.getpid()}"
+        torch.save({"m": o.W.m.cpu(), "e": o.W.e.cpu(), "inputs": torch.tensor(o.inputs),
+                    "outputs": torch.tensor(o.outputs)}, tmp)
+        os.replace(tmp, path)
+    except OSError:
+        pass
+    return o

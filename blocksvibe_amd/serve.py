@@ -32,7 +32,7 @@ import torch
 
 from . import _lib, mpf
 from .derivatives import Derivs
-from .output import Output
+from .output import cached as output_cached
 from .plan import Plan
 from .recursion import Recursion
 
@@ -196,7 +196,12 @@ class _Graph:
 
 
 class Server:
-    def __init__(self, hbm_budget=None, streams=2, max_batch=64, graphs=True, graph_budget=None):
+    def __init__(self, hbm_budget=None, streams=2, max_batch=64, graphs=True, graph_budget=None, window=0.0,
+                 map_cache=None):
+        """window: seconds a request that could be batched may wait for companions (requests of the same
+        operator from other callers, e.g. concurrent blocks_3d invocations) before its batch starts;
+        map_cache: directory keeping write_output maps across processes (output.cached)"""
+        self.window, self.map_cache = window, map_cache
         if hbm_budget is None:
             free, _ = torch.cuda.mem_get_info()
             hbm_budget = int(0.5 * free)
@@ -257,7 +262,7 @@ class Server:
         k = (coord, Lam, p0, d12, L)
         with self.setup_lock:
             if k not in self.outs:
-                self.outs[k] = Output(coord, Lam, p0, d12, L)
+                self.outs[k] = output_cached(coord, Lam, p0, d12, L, self.map_cache)
             return self.outs[k]
 
     def _take_batch(self):
@@ -289,6 +294,10 @@ class Server:
                             break
                         if r.key == head.key and r.sign == head.sign and len(r.hs) == len(head.hs):
                             batch.append(r)
+                    wait = self.window - (time.time() - head.t_arrival)
+                    if wait > 0 and len(batch) < nmax and not self.stop:
+                        self.lock.wait(min(wait, 0.05))           # room for companions to arrive
+                        continue
                     for r in batch:
                         self.queue.remove(r)
                     cost = per * len(batch)

@@ -205,7 +205,9 @@ def _number_words(rng, K, spread=2):
 def write_synthetic(path, precision, two_js, two_j12, two_j43, order, Lam, two_j_max, radial, q4s, seed=0,
                     pseudo_parities=None, skip_m=False):
     """a setup file with seeded random M matrices (every key the recursion needs) and h_inf entries
-    (every (pp, m <= order, j, a, b) the chain reads) for the tasks q4s = [(q4, sign)]"""
+    (every (pp, m <= order, j, a, b) the chain reads) for the tasks q4s = [(q4, sign)].  Each task's h_inf
+    comes from a generator seeded by (seed, sign, q4) alone, so it does not depend on skip_m or on the other
+    tasks of the file."""
     from .plan import Plan
     rng = np.random.default_rng(seed)
     rounded = ((precision - 1) // 64 + 1) * 64
@@ -235,6 +237,7 @@ def write_synthetic(path, precision, two_js, two_j12, two_j43, order, Lam, two_j
     for q4, sign in q4s:
         pp = pps[(tuple(q4), sign)]
         p0 = (1 - sign) // 2
+        hrng = np.random.default_rng([seed, sign + 1] + [int(q) + 64 for q in q4])
         ns = [p0] if radial else list(range(p0, Lam + 1, 2))
         words.append(np.array(list(q4) + [sign, pp[0][0], pp[0][1], pp[1][0], pp[1][1], len(ns)], np.int64))
         idx = []
@@ -249,9 +252,9 @@ def write_synthetic(path, precision, two_js, two_j12, two_j43, order, Lam, two_j
         for n in ns:
             words.append(np.array([n, len(idx)], np.int64))
             words.append(idx.reshape(-1))
-            vals = rng.integers(0, 1 << 62, size=(len(idx), K), dtype=np.int64)
+            vals = hrng.integers(0, 1 << 62, size=(len(idx), K), dtype=np.int64)
             vals[:, -1] |= np.int64(1) << 62
-            sizes = np.where(rng.integers(0, 2, size=len(idx)) == 1, K, -K)
+            sizes = np.where(hrng.integers(0, 2, size=len(idx)) == 1, K, -K)
             exps = np.zeros(len(idx), dtype=np.int64)
             words.append(np.concatenate([sizes[:, None], exps[:, None], vals], 1).reshape(-1))
     tmp = f"{path}.{os.getpid()}"

@@ -169,54 +169,3 @@ def test_select_and_errors():
     with pytest.raises(_lib.B3DError, match="too small"):
         _lib.call("b3d_format_polys", L, m.ctypes.data, e.ctypes.data, 4, 2, 2, 30, -900, buf.ctypes.data, 3,
                   end.ctypes.data, 1)
-
-
-def test_write_outputs_matches_python_writer(tmp_path):
-    """cli.write_outputs (bodies from b3d_format_polys) against the Python writer (fmt_exact and
-    trimmed_length per coefficient) on the same random results, file for file, byte for byte"""
-    from blocksvibe_amd import cli
-    from blocksvibe_amd import structure as S
-    args = ("--j-external 2,2,2,2 --j-12 2 --j-43 2 --delta-12 0 --delta-43 0 --j-internal 0-4 "
-            "--delta-1-plus-2 2.3 --four-pt-struct 1,2,1,0 --four-pt-sign 1 --order 8 --lambda 3 --coordinates xt "
-            f"--kept-pole-order 8 --precision 256 --num-threads 3 -o {tmp_path}/b_{{}}.json").split()
-    t = cli.Task(args)
-    L, pp = 10, ((0, 1), (1, 0))
-    tjs = list(t.two_js_internal)
-    Sm = max(len(S.so3(pp[k][w], (t.two_j12, t.two_j43)[w], tj)) for k in range(2) for w in range(2) for tj in tjs)
-    outputs = [(n, b) for n in range(4) for b in range((3 - n) // 2 + 1)]
-    Jo, S2, no, D = len(tjs), Sm * Sm, len(outputs), 5
-    rng = np.random.default_rng(4)
-    cnt = L * Jo * 2 * S2 * no * D
-    m = rng.integers(-2 ** 31, 2 ** 31, size=cnt, dtype=np.int64).astype(np.int32).reshape(L, Jo, 2, S2, no, D)
-    m[L - 1] = (m[L - 1] & 0x3FFFFFFF) | 0x20000000                   # normalised: bit length 32 L - 2
-    m[L - 1] *= np.where(rng.random(m.shape[1:]) < 0.5, -1, 1).astype(np.int32)
-    e = rng.integers(-330, -300, size=m.shape[1:]).astype(np.int64)
-    e[..., 3:] -= rng.choice([0, 280], size=e[..., 3:].shape)       # some negligible top coefficients
-    zero = rng.random(e.shape) < 0.15
-    zero[0, 0, 0, 0] = True                                           # one all-zero polynomial
-    m[:, zero] = 0
-    e[zero] = EZ
-    results = {(tuple(t.q4), t.sign): (tjs, pp, {("xt", t.delta_1_plus_2): (m, e)}, {("xt", t.delta_1_plus_2): outputs},
-                                       Sm)}
-    cli.write_outputs([t], results)
-    p = BJ.digits10(t.precision)
-    for jo, tj in enumerate(tjs):
-        structs = [(len(S.so3(pp[k][0], t.two_j12, tj)), len(S.so3(pp[k][1], t.two_j43, tj))) for k in range(2)]
-        flat = mpf.to_pairs(m[:, jo], e[jo])
-        rows_k = []
-        for k in range(2):
-            rows_a = []
-            for a in range(structs[k][0]):
-                rows_b = []
-                for b in range(structs[k][1]):
-                    rows = {}
-                    for o, (n, _) in enumerate(outputs):
-                        base = ((k * S2 + a * Sm + b) * no + o) * D
-                        rows.setdefault(n, []).append(flat[base:base + D])
-                    rows_b.append([rows[n] for n in sorted(rows)])
-                rows_a.append(rows_b)
-            rows_k.append(rows_a)
-        want = BJ.spin_text(t, tj, pp, structs, [rows_k])
-        got = open(t.path(tj / 2.0)).read()
-        assert got == want, tj
-    assert '"0"' in got

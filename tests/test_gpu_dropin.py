@@ -66,18 +66,32 @@ def read_setup(path):
     return M, tasks
 
 
-def test_dropin_batch_json(tmp_path):
+def cli_env(tmp_path, **extra):
     helper = tmp_path / "b3d_setup"
     helper.write_text(f"#!/bin/bash\nexec {sys.executable} {ROOT}/tests/synthetic_b3d_setup.py \"$@\"\n")
     helper.chmod(0o755)
+    env = dict(os.environ, BLOCKSVIBE_B3D_SETUP=str(helper), BLOCKSVIBE_VERBOSE="1",
+               BLOCKSVIBE_CACHE=str(tmp_path / "cache"), BLOCKSVIBE_SOCKET=f"/tmp/b3d-{os.getpid()}-{tmp_path.name}.sock",
+               BLOCKSVIBE_DAEMON_IDLE="20", PYTHONPATH=ROOT, **extra)
+    env.pop("BLOCKSVIBE_FALLBACK", None)
+    return env
+
+
+def stop(env):
+    from blocksvibe_amd import cli
+    cli.stop_daemon(env["BLOCKSVIBE_SOCKET"])
+
+
+def test_dropin_batch_json(tmp_path):
     lines = tmp_path / "lines.txt"
     lines.write_text("".join(f"{GROUP} --four-pt-struct {q} --four-pt-sign {s} -o {tmp_path}/out_{i}/b_{{}}.json\n"
                              for i, (q, s) in enumerate(TASKS)))
-    env = dict(os.environ, BLOCKSVIBE_B3D_SETUP=str(helper), BLOCKSVIBE_SPOOL_OFF="1", BLOCKSVIBE_VERBOSE="1",
-               BLOCKSVIBE_CACHE=str(tmp_path / "cache"), BLOCKSVIBE_LOCKDIR=str(tmp_path), PYTHONPATH=ROOT)
-    env.pop("BLOCKSVIBE_FALLBACK", None)
-    r = subprocess.run([os.path.join(ROOT, "bin", "blocks_3d"), "batch", "--lines", str(lines)], env=env,
-                       capture_output=True, text=True, timeout=600)
+    env = cli_env(tmp_path)
+    try:
+        r = subprocess.run([os.path.join(ROOT, "bin", "blocks_3d"), "batch", "--lines", str(lines)], env=env,
+                           capture_output=True, text=True, timeout=600)
+    finally:
+        stop(env)
     print(r.stderr[-3000:])
     assert r.returncode == 0
     # the same synthetic setup file, regenerated, gives the exact inputs
@@ -86,8 +100,8 @@ def test_dropin_batch_json(tmp_path):
     sfile = tmp_path / "setup.bin"
     write_synthetic(str(sfile), 256, [2, 2, 2, 2], 2, 2, 6, 3, 6, False, q4s, seed=1)
     M, stasks = read_setup(str(sfile))
+    mp.mp.prec = 400                          # before any mpmath value of the oracle is made
     order, Lam, kept, tj12, tj43, tjmax, D = 6, 3, 4, 2, 2, 6, mp.mpf("2.3")
-    mp.mp.prec = 400
     f2m = lambda x: mp.mpf(x.numerator) / x.denominator
     worst_all = mp.mpf(0)
     for i, ((q4, sign), (qs, _)) in enumerate(zip(q4s, TASKS)):
@@ -127,3 +141,35 @@ def test_dropin_batch_json(tmp_path):
     bits = -float(mp.log(worst_all, 2)) if worst_all else float("inf")
     print(f"drop-in JSON vs oracle chain: worst 2^-{bits:.1f} relative to each spin table")
     assert bits > 256 - 32
+
+
+def test_concurrent_invocations_share_a_batch(tmp_path):
+    """two blocks_3d processes started together with tasks of one operator: the daemon runs them as one device
+    batch, and their files equal those of the same tasks run alone in their own process (BLOCKSVIBE_DAEMON=0)"""
+    from blocksvibe_amd import cli
+    env = cli_env(tmp_path, BLOCKSVIBE_WINDOW="3")
+    args = [f"{GROUP} --four-pt-struct {q} --four-pt-sign {s}".split() for q, s in (TASKS[0], TASKS[2])]
+    try:
+        procs = [subprocess.Popen([os.path.join(ROOT, "bin", "blocks_3d")] + a + ["-o", f"{tmp_path}/c{i}/b_{{}}.json"],
+                                  env=env, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+                 for i, a in enumerate(args)]
+        outs = [p.communicate(timeout=600) for p in procs]
+        stats = cli.daemon_stats(env["BLOCKSVIBE_SOCKET"])
+    finally:
+        stop(env)
+    for p, (_, err) in zip(procs, outs):
+        print(err[-2000:])
+        assert p.returncode == 0
+    print("daemon batches:", stats["batch_sizes"])
+    assert 2 in stats["batch_sizes"]
+    alone = dict(env, BLOCKSVIBE_DAEMON="0")
+    for i, a in enumerate(args):
+        r = subprocess.run([os.path.join(ROOT, "bin", "blocks_3d")] + a + ["-o", f"{tmp_path}/a{i}/b_{{}}.json"],
+                           env=alone, capture_output=True, text=True, timeout=600)
+        assert r.returncode == 0, r.stderr[-2000:]
+        for tj in range(0, 7, 2):
+            got = open(tmp_path / f"c{i}" / f"b_{tj // 2}.json").read().splitlines()
+            want = open(tmp_path / f"a{i}" / f"b_{tj // 2}.json").read()
+            want = want.replace(f"{tmp_path}/a{i}/", f"{tmp_path}/c{i}/").splitlines()
+            first = next((k for k, (x, y) in enumerate(zip(got, want)) if x != y), None)
+            assert first is None and len(got) == len(want), (i, tj, first, got[first:first + 2], want[first:first + 2])
