@@ -12,7 +12,8 @@ import numpy as np
 
 
 def n_poles(tj, tj12, tj43, order):
-    """pole counts per type, int64 (4, *shape) (reference structure.py:16-24, clipped at 0)"""
+    """pole counts per type, int64 (4, *shape) (reference structure.py:16-24), clipped at 0 as the reference's
+    vectorised counts_v (structure.py:91-99); the recursion asks only at orders >= 1, where no count is negative"""
     tj = np.asarray(tj, dtype=np.int64)
     order = np.asarray(order, dtype=np.int64)
     tj, order = np.broadcast_arrays(tj, order)

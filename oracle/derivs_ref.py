@@ -6,8 +6,11 @@ residues_to_derivatives, restore_r_delta, shift_fraction, together_with_factors)
 shift_poles as described at reference blocksvibe/derivatives.py:27-33: the unprotected poles x_i != 0 are
 replaced by the kept ones x'_k with residues a' = A^-1 V a, A_rk = x'_k^(r-h), V_ri = x_i^(r-h), r < size,
 h = size // 2 -- solved here by exact Gaussian elimination over the rationals (independent of the closed
-form the product uses).  Pinned by tests/test_oracle_derivs.py (closed-form single-pole cases, the
-kept >= order identity, the LU solve against the Lagrange closed form).
+form the product uses).  Pinned by tests/test_oracle_pins.py: numerator_e(x) / prod (x - x_i) equals the closed
+form of the truncated radial series' e-th log r derivative at rational x off the poles (every step: the moment
+weights, restore_r_delta's division, the shift to x, together_with_factors), and, with pole shifting, the same
+identity over the kept poles when the other residues vanish; by tests/test_oracle_derivs.py: the LU solve
+against the Lagrange closed form.
 """
 from fractions import Fraction
 from math import comb

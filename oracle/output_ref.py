@@ -5,7 +5,10 @@ one_minus_power_prefactor) and reference tests/test_blocks_output.py:13-47 (cong
 lambda_lr_to_lrho_lrhob -> congruence transforms -> zzb_to_xt) plus the xt_radial product.
 Pinned by tests/test_oracle_output.py: Stirling numbers against their textbook values, rho_z against
 numerical z-derivatives of rho(z)^a at z = 1/2 (rho(1/2) = 3 - 2 sqrt 2), power_prefactor against
-numerical derivatives of the prefactor function.
+numerical derivatives of the prefactor function; and by tests/test_oracle_pins.py: the whole xt and xt_radial
+chains against numerical x, t derivatives of (z zbar)^(-D/2) h(rho(z), rho(zbar)) for known even and odd
+functions h, fed their numerical (log r, angle) derivatives (the odd part's orientation y = (zbar - z) / 2 is
+the one convention the mathematics leaves open).
 """
 import math
 

@@ -3,9 +3,10 @@
 Restates reference blocksvibe/plan_ref.py:9-132 with plain dicts and loops (the semantics the vectorised
 product Plan must reproduce): slot offsets, per-level (src, two_j') groups with their rows D' and prime-
 pole columns, tasks (slot, group, row, M keys, Q key), and the physical slot assignment with LIFO reuse.
-Pinned by tests/test_oracle_structure.py against the published slot counts (reference
-docs/BLOCKS_PLAN.md:209-210) and by the invariants every slot is written before it is read and no
-physical slot is reused while live.
+Pinned by tests/test_golden.py against the reference's own PlanRef on six operators with and without reuse
+(tests/golden/plan.json: keys, offsets, groups, Cauchy differences, tasks, physical slots), by
+tests/test_oracle_structure.py against the published slot counts (reference docs/BLOCKS_PLAN.md:209-210) and by
+the invariants every slot is written before it is read and no physical slot is reused while live.
 """
 from . import structure_ref as S
 

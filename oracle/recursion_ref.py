@@ -10,9 +10,10 @@ recursion_bits() -- the GPU kernels' bit-level semantics for the same loop, on t
 
 Inputs: h_inf[(pp, m, jidx, a, b)] and M[((t,k), delta, two_jNN, ppp, two_j)] -> matrix.
 Output: res[(p, m, jidx, pole_index, a, b)].
-Pinned by tests/test_oracle_recursion.py: the exact recursion against a brute-force evaluation of the
-Cauchy/pole formulas on tiny operators and against closed-form single-level cases; recursion_bits against
-recursion() to its error bound.
+Pinned by tests/test_golden.py: recursion() equals, residue for residue, the reference's own
+recursion_ref.recursion on eight seeded operators (tests/golden/recursion.json, written by
+tests/golden/make_fixtures.py from the reference's module); by tests/test_oracle_recursion.py: recursion_bits
+against recursion() to its error bound.
 """
 from fractions import Fraction
 

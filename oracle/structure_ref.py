@@ -2,8 +2,9 @@
 
 Restates reference blocksvibe/structure.py:16-83 (a port of blocks_3d's Pole.hxx, Pole_Types.cxx,
 two_j_values.cxx, so3_tensor_parity.cxx, fill_residues.cxx).  Spins doubled; pole = (type 0..3, k >= 1).
-Pinned by tests/test_oracle_structure.py: the published task / slot counts (reference
-docs/BLOCKS_PLAN.md:116-123,209-210), and closed forms of the pole positions.
+Pinned by tests/test_golden.py (every function against vectors written from the reference's structure.py
+itself, tests/golden/structure.json) and tests/test_oracle_structure.py: the published task / slot counts
+(reference docs/BLOCKS_PLAN.md:116-123,209-210), and closed forms of the pole positions.
 """
 from fractions import Fraction
 from functools import lru_cache
@@ -27,8 +28,11 @@ def poles(tj, tj12, tj43, order):
 
 
 def position(tj, tj12, tj43, order, pole):
+    """index of the pole in poles(tj, tj12, tj43, order); the counts of the lower types are summed as they are,
+    negative ones included (reference structure.py:31-33; the recursion asks for positions at orders where a
+    count can be negative, reference recursion_ref.py:27)"""
     t, k = pole
-    return k - 1 + sum(max(count(u, tj, tj12, tj43, order), 0) for u in range(t))
+    return k - 1 + sum(count(u, tj, tj12, tj43, order) for u in range(t))
 
 
 def shift(pole):

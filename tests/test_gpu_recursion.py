@@ -1,6 +1,8 @@
 """Batched GPU radial recursion (blocksvibe_amd.recursion) against the oracle, on random operators and
 initial data with several right-hand sides (reference tests/test_blocks_recursion.py:12-52):
-  * bit for bit against oracle.recursion_ref.recursion_bits (the kernels' exact rounding semantics);
+  * bit for bit against oracle.recursion_ref.recursion_bits (the kernels' exact rounding semantics), with the
+    MFMA pole step where it applies and again with the VALU pole step and two Cauchy rows per lane on every level;
+  * against the reference's own recursion vectors (tests/golden/recursion.json) within 2^-(32L - 64);
   * within 2^-(32L - 64) of the exact rational recursion relative to each (pole, spin) block;
   * with slot recycling (reuse=True) every residue, captured right after its level, is identical.
 Residues hold SL x SR structure entries (left and right structure counts; asymmetric cases 3 x 2, 2 x 1,
@@ -30,7 +32,10 @@ CASES = [dict(tj12=4, tj43=2, tjmax=6, order=7, L=10, B=3, seed=1, pp=((0, 1), (
          # 5 x 5, B 7: 35 rows per constant row, the pole step on int8 MFMA
          dict(tj12=8, tj43=8, tjmax=4, order=6, L=10, B=7, seed=7, pp=((0, 1), (1, 0))),
          # 4 x 2, B 8: 64 columns per row, the row-uniform Cauchy kernel
-         dict(tj12=6, tj43=2, tjmax=4, order=5, L=34, B=8, seed=5, pp=((0, 1), (1, 0)))]
+         dict(tj12=6, tj43=2, tjmax=4, order=5, L=34, B=8, seed=5, pp=((0, 1), (1, 0))),
+         # 5 x 5 at L = 34 (the heaviest operator group's structures and limbs), B 2: the VALU pole kernels and the
+         # MFMA ones against the oracle directly
+         dict(tj12=8, tj43=8, tjmax=2, order=5, L=34, B=2, seed=8, pp=((0, 1), (1, 0)))]
 
 
 def random_operator(plan, tj12, tj43, pp, order, B, seed):
@@ -82,6 +87,11 @@ def test_recursion_bits_and_exact(case):
     rec.mma = True                       # the int8 MFMA pole step where SL = SR = S, S B >= 32, the VALU one elsewhere
     res = rec.run(h)
     got = res.to_host()
+    # the same recursion with the VALU pole step everywhere and two Cauchy rows per lane on every level
+    # (k_cauchy_pairs, which the default threshold only selects on levels that fill the chip)
+    rec_v = Recursion(plan, M, L)
+    rec_v.mma, rec_v.pair_min_waves = False, 0
+    got_v = rec_v.run(h).to_host()
     # slot reuse: capture the slots written at each level right after it
     rec_r = Recursion(plan_r, M, L)
     cap = {}
@@ -109,6 +119,7 @@ def test_recursion_bits_and_exact(case):
             slot = rec.slot(p, m, j, q)
             g = got[(slot * S2 + a * SR + b) * B + k]
             assert g == v, ("bits", k, (p, m, j, q, a, b), g, v)
+            assert got_v[(slot * S2 + a * SR + b) * B + k] == v, ("bits, VALU pole step, paired Cauchy rows", k)
             ex = exact.get((p, m, j, q, a, b))
             if ex is not None and blk[(p, m, j, q)]:
                 worst = max(worst, abs(R.value(*g) - ex) / blk[(p, m, j, q)])
@@ -146,3 +157,37 @@ def test_cauchy_kernels_agree(case):
     a = outs[0]
     for b in outs[1:]:
         assert torch.equal(a.e, b.e) and torch.equal(a.m, b.m)
+
+
+@pytest.mark.parametrize("i,L", [(0, 10), (1, 10), (2, 10), (3, 10), (4, 34), (5, 10), (6, 10), (7, 18)])
+def test_recursion_golden_vectors(i, L):
+    """the GPU recursion on the inputs of the reference's recursion vectors (written by reference
+    blocksvibe/recursion_ref.py, tests/golden/make_fixtures.py): every residue within 2^-(32L - 64) of the
+    vector, relative to its (pole, spin) block, as reference tests/test_blocks_recursion.py:45-52 bounds it"""
+    import json
+    import os
+    from blocksvibe_amd.plan import Plan
+    from blocksvibe_amd.recursion import Recursion
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "recursion.json")) as f:
+        case = json.load(f)["data"][i]
+    tj12, tj43, tjmax, order, pp = case["case"]
+    pp = tuple(tuple(x) for x in pp)
+    M = {((k[0][0], k[0][1]), k[1], k[2], k[3], k[4]): [[F(x) for x in row] for row in k[5]] for k in case["M"]}
+    hd = {tuple(x[:5]): F(x[5]) for x in case["h_inf"]}
+    want = {tuple(x[:6]): F(x[6]) for x in case["res"]}
+    plan = Plan(tj12, tj43, tjmax, order, pp, reuse=False)
+    print(f"golden recursion {case['case']} L={L}: {len(want)} residues", flush=True)
+    rec = Recursion(plan, M, L)
+    got = rec.run(h_tensor([hd], plan, L)).to_host()
+    S2, SR = plan.S2, plan.SRm
+    blk = {}
+    for key, v in want.items():
+        blk[key[:4]] = max(blk.get(key[:4], 0), abs(v))
+    worst = F(0)
+    for (p, m, j, q, a, b), v in want.items():
+        g = got[rec.slot(p, m, j, q) * S2 + a * SR + b]
+        if blk[(p, m, j, q)]:
+            worst = max(worst, abs(R.value(*g) - v) / blk[(p, m, j, q)])
+    bits = (worst.denominator.bit_length() - worst.numerator.bit_length()) if worst else float("inf")
+    print(f"max error relative to the (pole, spin) block: 2^-{bits} (bound 2^-{32 * L - 64})")
+    assert bits > 32 * L - 64
