@@ -8,6 +8,7 @@
  * m[i * L .. i * L + L - 1].
  */
 #include <pthread.h>
+#include <stdlib.h>
 #include <stdint.h>
 #include <string.h>
 #include <time.h>
@@ -284,4 +285,158 @@ double oc_time(int kind, int L, int K, int nthreads, double seconds) {
     if (jobs[i].elapsed > el) el = jobs[i].elapsed;
   }
   return units > 0 ? el / units : 0.0;
+}
+
+/* ------------------------------------------------------------------ the recursion, one level at a time */
+/* The same level loop as the GPU library's b3d_cauchy_level / b3d_pole_level, restated on the host over the
+ * same index tables (mpf_ref semantics: recursion_ref.recursion_bits), split over nthreads threads.  Arrays are
+ * entry-major; a residue slot holds N = S2 B entries, entry (a SR + b) B + k. */
+
+/* mpf_ref.align of K entries m[i * stride] (entry units): magnitudes shifted right onto the largest exponent,
+ * truncated, sign kept; returns that exponent (OC_EZ if every entry is zero) */
+static int64_t align_vec(const u32 *m, const int64_t *e, int64_t stride, int K, int L, u32 *out) {
+  int64_t E = OC_EZ;
+  int any = 0;
+  for (int i = 0; i < K; ++i) {
+    const u32 *x = m + (size_t)(i * stride) * L;
+    if (bitlen(x, L) == 0) continue;
+    if (!any || e[i * stride] > E) E = e[i * stride];
+    any = 1;
+  }
+  memset(out, 0, (size_t)K * L * 4);
+  if (!any) return OC_EZ;
+  u32 a[OC_MAXL];
+  for (int i = 0; i < K; ++i) {
+    const u32 *x = m + (size_t)(i * stride) * L;
+    if (bitlen(x, L) == 0) continue;
+    memcpy(a, x, (size_t)L * 4);
+    int neg = is_neg(a, L);
+    if (neg) negate(a, L);
+    int64_t s = E - e[i * stride];
+    if (s >= 32 * L) continue;
+    shr(a, L, (int)s, out + (size_t)i * L, L);
+    if (neg) negate(out + (size_t)i * L, L);
+  }
+  return E;
+}
+
+typedef struct {
+  int L, Kc, S, SL, SR, nthreads, tid;
+  int64_t N, B, units;
+  const u32 *rm, *hm, *Hm, *Rm, *Qm;
+  const int64_t *re, *he, *He, *Re, *Qe, *ghrow, *col, *thrw, *tslot;
+  const int32_t *rg, *rd, *cold, *ncol, *tR, *tQL;
+  u32 *om, *resm;
+  int64_t *oe, *rese;
+} lev_t;
+
+static void *cauchy_worker(void *arg) {
+  lev_t *j = (lev_t *)arg;
+  const int L = j->L, Kc = j->Kc;
+  const int64_t N = j->N;
+  u32 *xs = (u32 *)malloc((size_t)(Kc > 0 ? Kc : 1) * L * 4);
+  int64_t *xe = (int64_t *)malloc((size_t)(Kc > 0 ? Kc : 1) * 8);
+  int32_t *ds = (int32_t *)malloc((size_t)(Kc > 0 ? Kc : 1) * 4);
+  for (int64_t u = j->tid; u < j->units; u += j->nthreads) {
+    const int64_t h = u / N, c = u % N;
+    const int g = j->rg[h];
+    const int K = j->ncol[g];
+    for (int i = 0; i < K; ++i) {
+      const int64_t src = j->col[(int64_t)g * Kc + i] * N + c;
+      memcpy(xs + (size_t)i * L, j->rm + (size_t)src * L, (size_t)L * 4);
+      xe[i] = j->re[src];
+      ds[i] = j->rd[h] - j->cold[(int64_t)g * Kc + i];
+    }
+    const int64_t hi = j->ghrow[g] * N + c;
+    j->oe[h * N + c] = oc_cauchy(xs, xe, ds, K, L, j->hm + (size_t)hi * L, j->he[hi], j->om + (size_t)(h * N + c) * L);
+  }
+  free(xs);
+  free(xe);
+  free(ds);
+  return 0;
+}
+
+static void run_threads(lev_t *base, void *(*fn)(void *)) {
+  int n = base->nthreads < 1 ? 1 : (base->nthreads > 256 ? 256 : base->nthreads);
+  pthread_t th[256];
+  lev_t jobs[256];
+  for (int i = 0; i < n; ++i) {
+    jobs[i] = *base;
+    jobs[i].nthreads = n;
+    jobs[i].tid = i;
+    pthread_create(&th[i], 0, fn, &jobs[i]);
+  }
+  for (int i = 0; i < n; ++i) pthread_join(th[i], 0);
+}
+
+/* H (nH, N) of one level: H[h][c] = cauchy(res[col[g][i]][c] for i < ncol[g], rd[h] - cold[g][i], hinf[ghrow[g]][c]),
+ * g = rg[h] (b3d_cauchy_level's contract) */
+int oc_cauchy_level(int L, int64_t N, const u32 *rm, const int64_t *re, const u32 *hm, const int64_t *he,
+                    const int64_t *ghrow, const int32_t *rg, const int32_t *rd, int64_t nH, const int64_t *col,
+                    const int32_t *cold, const int32_t *ncol, int Kc, u32 *Hm, int64_t *He, int nthreads) {
+  if (L < 1 || L > OC_MAXL) return 2;
+  lev_t j;
+  memset(&j, 0, sizeof(j));
+  j.L = L; j.N = N; j.rm = rm; j.re = re; j.hm = hm; j.he = he; j.ghrow = ghrow; j.rg = rg; j.rd = rd;
+  j.col = col; j.cold = cold; j.ncol = ncol; j.Kc = Kc; j.om = Hm; j.oe = He; j.nthreads = nthreads;
+  j.units = nH * N;
+  run_threads(&j, cauchy_worker);
+  return 0;
+}
+
+static void *pole_worker(void *arg) {
+  lev_t *j = (lev_t *)arg;
+  const int L = j->L, S = j->S, SL = j->SL, SR = j->SR;
+  const int64_t B = j->B, N = (int64_t)SL * SR * B;
+  u32 *hal = (u32 *)malloc((size_t)SL * SR * L * 4), *y = (u32 *)malloc((size_t)SR * SL * L * 4);
+  u32 *yal = (u32 *)malloc((size_t)SL * L * 4);
+  int64_t *hae = (int64_t *)malloc((size_t)SL * 8), *ye = (int64_t *)malloc((size_t)SR * SL * 8);
+  for (int64_t u = j->tid; u < j->units; u += j->nthreads) {
+    const int64_t t = u / B, k = u % B;
+    const int64_t hrow = j->thrw[t] * N;
+    /* the H operand: for each a, the entries b < SR at column k, aligned */
+    for (int a = 0; a < SL; ++a) {
+      const int64_t base = hrow + ((int64_t)a * SR) * B + k;
+      hae[a] = align_vec(j->Hm + (size_t)base * L, j->He + base, B, SR, L, hal + (size_t)a * SR * L);
+    }
+    /* Y[b0][a] = R[b0] . H[a] */
+    const int64_t rt = (int64_t)j->tR[t] * S;
+    for (int b0 = 0; b0 < SR; ++b0)
+      for (int a = 0; a < SL; ++a)
+        ye[b0 * SL + a] = oc_dot_aligned(j->Rm + (size_t)((rt + b0) * S) * L, j->Re[rt + b0], hal + (size_t)a * SR * L,
+                                         hae[a], SR, L, y + (size_t)(b0 * SL + a) * L);
+    /* res[a0][b0] = qL[a0] . Y[b0] */
+    const int64_t qt = (int64_t)j->tQL[t] * S;
+    const int64_t out = j->tslot[t] * N;
+    for (int b0 = 0; b0 < SR; ++b0) {
+      const int64_t E = align_vec(y + (size_t)b0 * SL * L, ye + b0 * SL, 1, SL, L, yal);
+      for (int a0 = 0; a0 < SL; ++a0) {
+        const int64_t o = out + ((int64_t)a0 * SR + b0) * B + k;
+        j->rese[o] = oc_dot_aligned(j->Qm + (size_t)((qt + a0) * S) * L, j->Qe[qt + a0], yal, E, SL, L,
+                                    j->resm + (size_t)o * L);
+      }
+    }
+  }
+  free(hal);
+  free(y);
+  free(yal);
+  free(hae);
+  free(ye);
+  return 0;
+}
+
+/* the pole step of one level (b3d_pole_level's contract): for task t, Y = R[tR[t]] . H[thrw[t]]^T, then
+ * res[tslot[t]] = qL[tQL[t]] . Y^T; R and qL tables (ntab, S, S) entry-major with row exponents (ntab, S),
+ * already aligned per row */
+int oc_pole_level(int L, int S, int SL, int SR, int64_t B, int64_t T, const u32 *Hm, const int64_t *He,
+                  const int64_t *thrw, const int32_t *tR, const int32_t *tQL, const int64_t *tslot, const u32 *Rm,
+                  const int64_t *Re, const u32 *Qm, const int64_t *Qe, u32 *rm, int64_t *re, int nthreads) {
+  if (L < 1 || L > OC_MAXL || SL < 1 || SR < 1 || SL > S || SR > S) return 2;
+  lev_t j;
+  memset(&j, 0, sizeof(j));
+  j.L = L; j.S = S; j.SL = SL; j.SR = SR; j.B = B; j.Hm = Hm; j.He = He; j.thrw = thrw; j.tR = tR; j.tQL = tQL;
+  j.tslot = tslot; j.Rm = Rm; j.Re = Re; j.Qm = Qm; j.Qe = Qe; j.resm = rm; j.rese = re; j.nthreads = nthreads;
+  j.units = T * B;
+  run_threads(&j, pole_worker);
+  return 0;
 }
