@@ -436,6 +436,19 @@ def serving_stream(seconds, outdir, seed=7):
             "latency_to": "last spin file closed", "files_written": emitter.files}
 
 
+def fs_type(path):
+    """the file system type holding path (from /proc/mounts: the longest mount point that prefixes it)"""
+    best, kind = "", "unknown"
+    try:
+        for line in open("/proc/mounts"):
+            f = line.split()
+            if len(f) > 2 and os.path.realpath(path).startswith(f[1].rstrip("/") + "/") and len(f[1]) > len(best):
+                best, kind = f[1], f[2]
+    except OSError:
+        pass
+    return kind
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -463,6 +476,7 @@ def main():
     torch.cuda.synchronize()
     blocks = 0
     per_step = []
+    files0, bytes0, write0, wait0 = emitter.files, emitter.bytes, emitter.write_s, emitter.wait_s
     t0 = time.time()
     for s in range(args.steps):
         ts = time.time()
@@ -480,10 +494,13 @@ def main():
                       "blocks_per_pass": sum(b.blocks() for b in batches), "steps_per_pass": len(bins),
                       "blocks_timed": blocks,
                       "step": "chain + GPU JSON text + copy + spin files written and closed"},
-           "end_to_end": {"files": emitter.files, "bytes_written": emitter.bytes,
-                          "gb_per_s_written": emitter.bytes / 1e9 / max(dt, 1e-9), "host_fallback_units":
-                              emitter.fallback_units, "step_s_min": min(per_step), "step_s_max": max(per_step),
-                          "constants_setup_s": setup_s}}
+           "end_to_end": {"files": emitter.files - files0, "bytes_written": emitter.bytes - bytes0,
+                          "gb_per_s_written": (emitter.bytes - bytes0) / 1e9 / max(dt, 1e-9),
+                          "host_fallback_units": emitter.fallback_units, "step_s_min": min(per_step),
+                          "step_s_max": max(per_step), "constants_setup_s": setup_s,
+                          "writer_busy_s": emitter.write_s - write0, "writers": args.writers,
+                          "chain_waited_for_writers_s": emitter.wait_s - wait0, "out_dir": outdir,
+                          "out_fs": fs_type(outdir)}}
     emitter.close()
     log(f"end to end: {res['value']:.1f} blocks/s")
     if not args.no_device_pass:

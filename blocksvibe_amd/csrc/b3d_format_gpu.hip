@@ -46,6 +46,23 @@ __device__ __forceinline__ bool lt_abs(const int32_t* m, i64 ps, i64 i, i64 ei, 
   return false;
 }
 
+// The top limb t of a normalised value (magnitude bit length 32 L - 2) whose bit length it alone certifies:
+// t in [2^29, 2^30) (positive) or t in (-2^30, -2^29) (negative; the end points also occur for values of other
+// bit lengths).
+__device__ __forceinline__ bool top_certifies(u32 t) {
+  const int32_t v = (int32_t)t;
+  return (v >= (1 << 29) && v < (1 << 30)) || (v > -(1 << 30) && v < -(1 << 29));
+}
+
+// lt_abs with the exponents first: when both top limbs certify the bit length 32 L - 2, |x| 2^ex < |y| 2^ey is
+// decided by ex < ey unless ex == ey; only then (or for an uncertified value) are the limbs compared.  ti: the
+// top limb of entry i, already loaded.
+template <int L>
+__device__ __forceinline__ bool lt_abs_fast(const int32_t* m, i64 ps, i64 i, u32 ti, i64 ei, i64 j, u32 tj, i64 ej) {
+  if (top_certifies(ti) && top_certifies(tj) && ei != ej) return ei < ej;
+  return lt_abs<L>(m, ps, i, ei, j, ej);
+}
+
 }  // namespace
 
 template <int L>
@@ -56,14 +73,17 @@ __global__ __launch_bounds__(256) void k_fmt_sizes(const int32_t* __restrict__ m
     const i64 base = pbase[q];
     int size = 0;
     i64 mx = -1, emx = 0;
+    u32 tmx = 0u;
     for (int d = deg - 1; d >= 0; --d) {
       const i64 i = base + d;
       const i64 ei = e[i];
       if (ei == B3D_EZ) continue;   // canonical zero: 0 < 2^eps 0 is false
-      if (mx < 0 || lt_abs<L>(m, ps, mx, emx, i, ei + eps)) size = d + 1;
-      if (mx < 0 || lt_abs<L>(m, ps, mx, emx, i, ei)) {
+      const u32 ti = (u32)m[(i64)(L - 1) * ps + i];
+      if (mx < 0 || lt_abs_fast<L>(m, ps, mx, tmx, emx, i, ti, ei + eps)) size = d + 1;
+      if (mx < 0 || lt_abs_fast<L>(m, ps, mx, tmx, emx, i, ti, ei)) {
         mx = i;
         emx = ei;
+        tmx = ti;
       }
     }
     sizes[q] = size;
@@ -243,6 +263,72 @@ __global__ __launch_bounds__(256) void k_fmt_pack(const uint8_t* __restrict__ tx
   }
 }
 
+// Packed files with coalesced stores: a block takes blockDim.x consecutive elements, whose pieces are one
+// contiguous range of out; each thread lays its piece out in LDS (at its offset in the range, shifted so that LDS
+// and global addresses agree mod 4), then the block stores the range as aligned words (bytes at its two ends,
+// which it shares with its neighbours).  A block whose range exceeds the LDS buffer (cap bytes; long header
+// strings) writes each piece straight to global memory as k_fmt_pack does.
+__device__ __forceinline__ i64 piece_bytes(const int32_t* len, const uint8_t* last, const int32_t* pre, const int32_t* post,
+                                           const int64_t* stroff, i64 k) {
+  if (last[k] > 1) return 0;
+  i64 b = (len[k] > 0 ? len[k] : 0) + 2 + (last[k] ? 0 : 2);   // len < 0 (left to the host) writes no digits
+  if (pre[k] >= 0) b += stroff[pre[k] + 1] - stroff[pre[k]];
+  if (post[k] >= 0) b += stroff[post[k] + 1] - stroff[post[k]];
+  return b;
+}
+
+__device__ __forceinline__ void write_piece(const uint8_t* __restrict__ tx, const int32_t* __restrict__ len,
+                                            const uint8_t* __restrict__ last, const int32_t* __restrict__ pre,
+                                            const int32_t* __restrict__ post, const uint8_t* __restrict__ strtab,
+                                            const int64_t* __restrict__ stroff, i64 n, i64 k, uint8_t* o) {
+  if (pre[k] >= 0)
+    for (i64 c = stroff[pre[k]]; c < stroff[pre[k] + 1]; ++c) *o++ = strtab[c];
+  const int l = len[k];
+  *o++ = '"';
+  for (int c = 0; c < l; ++c) *o++ = tx[(i64)c * n + k];
+  *o++ = '"';
+  if (!last[k]) {
+    *o++ = ',';
+    *o++ = '\n';
+  }
+  if (post[k] >= 0)
+    for (i64 c = stroff[post[k]]; c < stroff[post[k] + 1]; ++c) *o++ = strtab[c];
+}
+
+__global__ __launch_bounds__(128) void k_fmt_pack_lds(const uint8_t* __restrict__ tx, const int32_t* __restrict__ len,
+                                                      const uint8_t* __restrict__ last, const int32_t* __restrict__ pre,
+                                                      const int32_t* __restrict__ post,
+                                                      const uint8_t* __restrict__ strtab,
+                                                      const int64_t* __restrict__ stroff, const int64_t* __restrict__ off,
+                                                      i64 n, uint8_t* __restrict__ out, int cap) {
+  extern __shared__ uint32_t sw[];
+  uint8_t* sb = (uint8_t*)sw;
+  const i64 k0 = (i64)blockIdx.x * blockDim.x;
+  const i64 k1 = (k0 + blockDim.x < n) ? k0 + blockDim.x : n;
+  const i64 k = k0 + threadIdx.x;
+  const i64 base = off[k0];
+  const i64 endb = off[k1 - 1] + piece_bytes(len, last, pre, post, stroff, k1 - 1);
+  const int sh = (int)(base & 3);
+  const i64 total = endb - base + sh;             // bytes of the LDS image, from the aligned word below base
+  if (total > cap) {
+    if (k < k1 && last[k] <= 1) write_piece(tx, len, last, pre, post, strtab, stroff, n, k, out + off[k]);
+    return;
+  }
+  if (k < k1 && last[k] <= 1) write_piece(tx, len, last, pre, post, strtab, stroff, n, k, sb + (off[k] - base + sh));
+  __syncthreads();
+  uint32_t* ow = (uint32_t*)(out + (base - sh));  // aligned: base - sh is a multiple of 4
+  const i64 nw = (total + 3) >> 2;
+  for (i64 w = threadIdx.x; w < nw; w += blockDim.x) {
+    const i64 lo = 4 * w, hi = lo + 4;
+    if (lo >= sh && hi <= total) {
+      ow[w] = sw[w];                              // a whole word of this block
+    } else {
+      for (i64 b = lo; b < hi; ++b)               // the partial words at the two ends
+        if (b >= sh && b < total) out[base - sh + b] = sb[b];
+    }
+  }
+}
+
 static inline unsigned grid_for(int64_t n, int th) {
   int64_t b = (n + th - 1) / th;
   if (b > (1 << 22)) b = 1 << 22;
@@ -296,7 +382,16 @@ int b3d_fmt_pack_files(const uint8_t* text, const int32_t* len, const uint8_t* l
     return 2;
   }
   hipStream_t s = (hipStream_t)stream;
-  k_fmt_pack<<<dim3(grid_for(n, 256)), dim3(256), 0, s>>>(text, len, last, pre, post, strtab, stroff, off, n, out);
+  // LDS image per block: 128 pieces of up to ~470 bytes (p + 20 at p = 309 is 329), within the 64 KiB a
+  // workgroup may allocate
+  constexpr int kThreads = 128, kCap = 60 * 1024;
+  const int64_t blocks = (n + kThreads - 1) / kThreads;
+  if (blocks > (int64_t(1) << 31) - 1) {
+    b3d::set_error("b3d_fmt_pack_files: too many elements");
+    return 2;
+  }
+  k_fmt_pack_lds<<<dim3((unsigned)blocks), dim3(kThreads), kCap, s>>>(text, len, last, pre, post, strtab, stroff, off,
+                                                                       n, out, kCap);
   return b3d::check_launch("b3d_fmt_pack_files");
 }
 

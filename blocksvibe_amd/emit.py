@@ -24,6 +24,7 @@ import concurrent.futures as cf
 import os
 import queue
 import threading
+import time
 
 import numpy as np
 
@@ -158,7 +159,7 @@ def _write_file(path, data, unlink):
 
 
 class _Unit:
-    __slots__ = ("jobs", "done", "buf", "meta", "m2", "e1", "D", "p", "eps", "ub", "empty", "call")
+    __slots__ = ("jobs", "ready", "packed", "dmeta", "buf", "meta", "m2", "e1", "D", "p", "eps", "ub", "empty", "call")
 
 
 class _Call:
@@ -188,7 +189,7 @@ class Emitter:
     the first error.  Thread-safe: several threads may emit, each on its own stream.  unlink: remove each file
     after writing it (throughput runs that must not fill the disk)."""
 
-    def __init__(self, writers=16, buffers=4, unit_slots=UNIT_SLOTS, unlink=False):
+    def __init__(self, writers=16, buffers=8, unit_slots=UNIT_SLOTS, unlink=False, pending=16):
         import torch
         self.torch = torch
         self.unit_slots = unit_slots
@@ -199,6 +200,8 @@ class Emitter:
         self.free = queue.Queue()
         self.made = 0
         self.buf_lock = threading.Lock()
+        # units whose bytes are still on the device (made, not yet copied): emit() waits beyond this many
+        self.pending = threading.BoundedSemaphore(max(1, pending))
         self.q = queue.Queue()
         self.errors = []
         self.lock = threading.Condition()
@@ -206,6 +209,9 @@ class Emitter:
         self.files = 0
         self.bytes = 0
         self.fallback_units = 0
+        self.wait_s = 0.0                 # time emit() callers waited for the copies and writes to catch up
+        self.write_s = 0.0                # summed time of the file writes over the writer threads
+        self.buffer_wait_s = 0.0          # time the copies waited for a free host buffer
         self.dirs = set()
         self.thread = threading.Thread(target=self._dispatch, daemon=True)
         self.thread.start()
@@ -266,7 +272,10 @@ class Emitter:
                 self.made += 1
         if make:
             return torch.empty(max(nbytes, 1 << 20), dtype=torch.uint8, pin_memory=True)
+        t0 = time.time()
         buf = self.free.get()
+        with self.lock:
+            self.buffer_wait_s += time.time() - t0
         if buf.numel() < nbytes:
             del buf
             buf = torch.empty(nbytes, dtype=torch.uint8, pin_memory=True)
@@ -277,14 +286,18 @@ class Emitter:
         from . import _lib
         u = _Unit()
         u.jobs, u.m2, u.e1, u.D, u.p, u.eps, u.call = jobs, m2, e1, D, p, eps, call
+        u.buf = None
         with self.lock:
             self.inflight += 1
         npoly = sum(len(j.polys) for j in jobs)
         u.empty = npoly == 0
         if u.empty:                                   # files without polynomials: header and closing text
-            u.buf = None
             self.q.put(u)
             return
+        t0 = time.time()
+        self.pending.acquire()
+        with self.lock:
+            self.wait_s += time.time() - t0
         # host tables: strings (per file: header + opening text; per skeleton: its texts), polynomial indices,
         # the string before each polynomial and after each file's last one
         strings, skel_base, pre, post, polys, first = [], {}, [], [], [], []
@@ -312,7 +325,6 @@ class Emitter:
         n = npoly * D
         ub = n * (p + 20) + int(slen[pre + 1].sum() + slen[post + 1].sum())
         u.ub = ub
-        u.buf = self._buffer(ub)
         dev = e1.device
         L = m2.shape[0]
         ps = e1.numel()
@@ -354,19 +366,9 @@ class Emitter:
         packed = torch.empty(ub, dtype=torch.uint8, device=dev)
         _lib.call("b3d_fmt_pack_files", P(tx), P(ln), P(last), P(pre_el), P(post_el), P(tstr), P(tsoff),
                   len(strings), P(off), n, P(packed), sptr)
-        meta = torch.cat([off[tfirst], end[-1:], bad])
-        ready = torch.cuda.Event()
-        ready.record(st)
-        u.meta = torch.empty(meta.numel(), dtype=torch.int64, pin_memory=True)
-        cs = self.copy_stream
-        cs.wait_event(ready)
-        with torch.cuda.stream(cs):
-            u.buf[:ub].copy_(packed, non_blocking=True)
-            u.meta.copy_(meta, non_blocking=True)
-        packed.record_stream(cs)
-        meta.record_stream(cs)
-        u.done = torch.cuda.Event()
-        u.done.record(cs)
+        u.packed, u.dmeta = packed, torch.cat([off[tfirst], end[-1:], bad])
+        u.ready = torch.cuda.Event()
+        u.ready.record(st)
         self.q.put(u)
 
     def _dispatch(self):
@@ -378,6 +380,9 @@ class Emitter:
                 self._finish(u)
             except BaseException as err:                  # surfaced by drain() and the call's on_done
                 self._release(u)
+                if getattr(u, "packed", None) is not None:
+                    u.packed = u.dmeta = None
+                    self.pending.release()
                 with self.lock:
                     self.errors.append(err)
                     self.inflight -= 1
@@ -389,7 +394,20 @@ class Emitter:
             datas = [_file_bytes(j, []) for j in u.jobs]
             self._write_all(u, [(j.path, x) for j, x in zip(u.jobs, datas)])
             return
-        u.done.synchronize()
+        # the copy: a host buffer (waiting here, not in emit(), when the writers are behind), then the bytes
+        torch = self.torch
+        u.buf = self._buffer(u.ub)
+        u.meta = torch.empty(u.dmeta.numel(), dtype=torch.int64, pin_memory=True)
+        cs = self.copy_stream
+        cs.wait_event(u.ready)
+        with torch.cuda.stream(cs):
+            u.buf[:u.ub].copy_(u.packed, non_blocking=True)
+            u.meta.copy_(u.dmeta, non_blocking=True)
+        done = torch.cuda.Event()
+        done.record(cs)
+        done.synchronize()
+        u.packed = u.dmeta = None
+        self.pending.release()
         meta = u.meta.numpy()
         F = len(u.jobs)
         starts, total, bad = meta[:F], int(meta[F]), int(meta[F + 1])
@@ -428,7 +446,11 @@ class Emitter:
 
         def one(path, data):
             try:
+                t0 = time.time()
                 nb = _write_file(path, data, self.unlink)
+                dt = time.time() - t0
+                with self.lock:
+                    self.write_s += dt
             except BaseException as err:
                 with self.lock:
                     self.errors.append(err)
