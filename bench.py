@@ -53,6 +53,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--serve", type=float, default=20.0, help="seconds of the mixed serving stream (0: skip)")
     ap.add_argument("--writers", type=int, default=16)
+    ap.add_argument("--streams", type=int, default=2, help="lanes of operator groups per step, one HIP stream each")
     ap.add_argument("--keep-files", default=None, help="write the files under this directory and keep them")
     ap.add_argument("--json-out", default=None)
     return ap.parse_args()
@@ -205,20 +206,65 @@ def make_bins(batches, nbins):
     return [x for x in bins if x]
 
 
-def step(bins, i, emitter, timer=None):
-    """one step: the chain and the spin files of bin i % len(bins); returns its blocks"""
+def lanes(batches, n):
+    """the batches of one step dealt into n lanes of about equal cost (largest first, each to the cheapest lane
+    so far).  Lanes may share an operator group: its constants are read-only once built, and those built on
+    first use are published only after their stream has finished them"""
+    out, load = [[] for _ in range(n)], [0.0] * n
+    for b in sorted(batches, key=lambda x: -x.cost()):
+        k = min(range(n), key=lambda j: load[j])
+        out[k].append(b)
+        load[k] += b.cost()
+    return [x for x in out if x]
+
+
+_STREAMS = []
+
+
+def step(bins, i, emitter, timer=None, streams=1):
+    """one step: the chain and the spin files of bin i % len(bins); returns its blocks.  streams > 1: the
+    step's batches in that many lanes, each lane on its own HIP stream driven by its own thread, so that one
+    lane's small levels share the GPU with another lane's work"""
+    import threading
     import torch
-    blocks = 0
-    for b in bins[i % len(bins)]:
-        r = b.run(timer=timer)
-        if emitter is not None:
-            b.emit(emitter, r)
-        blocks += b.blocks()
-        del r
+    batches = bins[i % len(bins)]
+    ls = lanes(batches, streams) if streams > 1 else [batches]
+
+    def run_lane(lane):
+        for b in lane:
+            r = b.run(timer=timer)
+            if emitter is not None:
+                b.emit(emitter, r)
+            del r
+
+    if len(ls) == 1:
+        run_lane(ls[0])
+    else:
+        while len(_STREAMS) < len(ls):
+            _STREAMS.append(torch.cuda.Stream())
+        main = torch.cuda.current_stream()
+        errors = []
+
+        def worker(k):
+            try:
+                s = _STREAMS[k]
+                s.wait_stream(main)
+                with torch.cuda.stream(s):
+                    run_lane(ls[k])
+            except BaseException as exc:            # re-raised on the calling thread
+                errors.append(exc)
+
+        th = [threading.Thread(target=worker, args=(k,)) for k in range(len(ls))]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        if errors:
+            raise errors[0]
     if emitter is not None:
         emitter.drain()
     torch.cuda.synchronize()
-    return blocks
+    return sum(b.blocks() for b in batches)
 
 
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_cauchy_nmax6_1024bit_pmc.txt")
@@ -471,7 +517,7 @@ def main():
     log(f"constants of {len(groups)} operator groups: {setup_s:.1f} s; {len(bins)} steps per pass")
     emitter = Emitter(writers=args.writers, buffers=4, unlink=args.keep_files is None)
     for w in range(args.warmup):
-        step(bins, w, emitter)
+        step(bins, w, emitter, streams=args.streams)
         log(f"warmup step {w} done")
     torch.cuda.synchronize()
     blocks = 0
@@ -480,7 +526,7 @@ def main():
     t0 = time.time()
     for s in range(args.steps):
         ts = time.time()
-        blocks += step(bins, args.warmup + s, emitter)
+        blocks += step(bins, args.warmup + s, emitter, streams=args.streams)
         per_step.append(time.time() - ts)
         log(f"step {s} done at {time.time() - t0:.1f} s ({per_step[-1]:.2f} s)")
     torch.cuda.synchronize()
@@ -499,6 +545,7 @@ def main():
                           "host_fallback_units": emitter.fallback_units, "step_s_min": min(per_step),
                           "step_s_max": max(per_step), "constants_setup_s": setup_s,
                           "writer_busy_s": emitter.write_s - write0, "writers": args.writers,
+                          "streams": args.streams,
                           "chain_waited_for_writers_s": emitter.wait_s - wait0, "out_dir": outdir,
                           "out_fs": fs_type(outdir)}}
     emitter.close()

@@ -16,6 +16,7 @@ S a, S_ki = (y'_k / y_i)^h l_k(y_i) (closed form of blocks_3d's A^-1 V, Lagrange
 h = size // 2), folded into the numerator matrix as exact rationals, rounded once.
 """
 import copy
+import threading
 from fractions import Fraction
 from math import comb, lcm
 
@@ -25,6 +26,8 @@ import torch
 
 from . import _lib, mpf, rns
 from . import structure as S
+
+_ORDERS_LOCK = threading.Lock()
 
 
 def _polymul_lin(p, a):
@@ -357,8 +360,19 @@ class Derivs:
         """the same stages for a smaller R (a higher lambda order of the task): the constants are built once
         per R and cached; each call returns its own object for the run state (start / accumulate / finish)"""
         assert R <= self.R
-        if R in self._orders:
-            return copy.copy(self._orders[R])       # shared constants, own run state (start/accumulate)
+        d = self._orders.get(R)
+        if d is None:
+            # several streams may ask at once: one builds, and its stream is synchronised before the constants
+            # are shared
+            with _ORDERS_LOCK:
+                d = self._orders.get(R)
+                if d is None:
+                    d = self._build_order(R)
+                    torch.cuda.current_stream().synchronize()
+                    self._orders[R] = d
+        return copy.copy(d)                          # shared constants, own run state (start/accumulate)
+
+    def _build_order(self, R):
         d = object.__new__(Derivs)
         d.__dict__.update(self.__dict__)
         d.R = R
@@ -380,5 +394,4 @@ class Derivs:
                                          dtype=np.uint32).view(np.int32)).to(mpf.dev())
         d.NKb = torch.tensor([x.bit_length() for x in v], dtype=torch.int32, device=mpf.dev())
         d._finish_consts()
-        self._orders[R] = d
-        return copy.copy(d)
+        return d

@@ -11,6 +11,7 @@ structure counts), not a padded Smax x Smax square.  Constants (M matrices, Q va
 forms) are built once per operator and reused by every run().
 """
 import os
+import threading
 import time
 from fractions import Fraction
 
@@ -19,6 +20,8 @@ import torch
 
 from . import _lib, mpf
 from . import structure as S
+
+_WAVES_LOCK = threading.Lock()
 
 
 def number_pair(x, L):
@@ -169,26 +172,35 @@ class Recursion:
         padded at the end of each group -> (first row of the group, rows of the group, first unit, waves,
         rows per lane): two when the level has at least pair_min_waves waves of pairs"""
         key = (N, self.pair_min_waves)
-        if key not in self._waves:
-            d, tabs = mpf.dev(), []
-            for lev in self.plan.levels:
-                if lev is None:
-                    tabs.append(None)
-                    continue
-                rg = lev.rg
-                starts = np.flatnonzero(np.r_[True, rg[1:] != rg[:-1]])
-                assert len(starts) == len(np.unique(rg)), "a level's rows must be grouped"
-                cnts = np.diff(np.r_[starts, len(rg)])
-                rpl = 2 if int((((cnts + 1) // 2 * N + 63) // 64).sum()) >= self.pair_min_waves else 1
-                nwg = ((cnts + rpl - 1) // rpl * N + 63) // 64
-                wrow = np.repeat(starts, nwg).astype(np.int32)
-                wcnt = np.repeat(cnts, nwg).astype(np.int32)
-                woff = ((np.arange(int(nwg.sum())) - np.repeat(np.cumsum(nwg) - nwg, nwg)) * 64).astype(np.int32)
-                assert cnts.max() * N < 1 << 31
-                tabs.append((torch.from_numpy(wrow).to(d), torch.from_numpy(wcnt).to(d), torch.from_numpy(woff).to(d),
-                             len(wrow), rpl))
-            self._waves[key] = tabs
-        return self._waves[key]
+        tabs = self._waves.get(key)
+        if tabs is not None:
+            return tabs
+        with _WAVES_LOCK:         # several streams may ask at once: built once, its stream synchronised
+            if key not in self._waves:
+                tabs = self._wave_tables(N)
+                torch.cuda.current_stream().synchronize()
+                self._waves[key] = tabs
+            return self._waves[key]
+
+    def _wave_tables(self, N):
+        d, tabs = mpf.dev(), []
+        for lev in self.plan.levels:
+            if lev is None:
+                tabs.append(None)
+                continue
+            rg = lev.rg
+            starts = np.flatnonzero(np.r_[True, rg[1:] != rg[:-1]])
+            assert len(starts) == len(np.unique(rg)), "a level's rows must be grouped"
+            cnts = np.diff(np.r_[starts, len(rg)])
+            rpl = 2 if int((((cnts + 1) // 2 * N + 63) // 64).sum()) >= self.pair_min_waves else 1
+            nwg = ((cnts + rpl - 1) // rpl * N + 63) // 64
+            wrow = np.repeat(starts, nwg).astype(np.int32)
+            wcnt = np.repeat(cnts, nwg).astype(np.int32)
+            woff = ((np.arange(int(nwg.sum())) - np.repeat(np.cumsum(nwg) - nwg, nwg)) * 64).astype(np.int32)
+            assert cnts.max() * N < 1 << 31
+            tabs.append((torch.from_numpy(wrow).to(d), torch.from_numpy(wcnt).to(d), torch.from_numpy(woff).to(d),
+                         len(wrow), rpl))
+        return tabs
 
     def run(self, h_inf, hook=None, prof=None, res=None, timer=None):
         """h_inf: MPF (2, order + 1, Jh, SL SR, B) (structure entry (a, b) at a SR + b).  Returns the residue

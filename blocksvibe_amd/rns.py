@@ -21,6 +21,7 @@ from . import _lib, mpf
 
 _PRIMES = None
 _TABLES = {}
+_TABLE_LOCK = threading.Lock()
 WORKSPACE_BYTES = 1 << 31
 
 # MFMA block tiles of b3d_rns_gemm (16 wm rows x 64 / wm columns).  With BLOCKSVIBE_RNS_TUNE=on the first GEMM
@@ -178,9 +179,21 @@ def crt_digits(ps, L):
 
 
 def tables(L, nmod):
+    """the device tables of (L, nmod), built once per process.  Callers on several streams share them: they are
+    built under a lock and the building stream is synchronised before they are published"""
     key = (L, nmod)
-    if key in _TABLES:
+    t = _TABLES.get(key)
+    if t is not None:
+        return t
+    with _TABLE_LOCK:
+        if key not in _TABLES:
+            t = _build_tables(L, nmod)
+            torch.cuda.current_stream().synchronize()
+            _TABLES[key] = t
         return _TABLES[key]
+
+
+def _build_tables(L, nmod):
     ps = primes_desc()[:nmod]
     M = 1
     for p in ps:
@@ -199,7 +212,6 @@ def tables(L, nmod):
          "inv": torch.from_numpy(inv.view(np.int16)).to(d),
          "rp": torch.from_numpy(rp).to(d),
          "c8": torch.from_numpy(c8).to(d), "CK": int(c8.shape[1])}
-    _TABLES[key] = t
     return t
 
 
