@@ -302,11 +302,27 @@ def roofline(timer):
             "note": "VALU-bound (one-limb long divisions): see profiles/ for VALU issue utilisation"}
 
 
-def cpu_baseline(groups, seconds=12.0):
-    """The oracle's C restatement of the recursion arithmetic (oracle/c/mpf_c.c: Cauchy entries and pole-step
-    dot products at 34 limbs, bit-identical to the GPU kernels' spec), timed on every host thread of this process
-    on a bounded sample, scaled by the workload's exact entry and term counts.  Derivative, output and text
+def _random_pairs(rng, n, L, spread):
+    """n random normalised (mantissa, exponent) pairs at L limbs, exponents within +-spread of 2^0"""
+    nb = 32 * L - 2
+    out = []
+    for _ in range(n):
+        m = int(rng.integers(1 << 62)) << (nb - 63) | (1 << (nb - 1))
+        out.append((m if rng.random() < .5 else -m, int(rng.integers(-spread, spread + 1)) - nb))
+    return out
+
+
+def cpu_baseline(groups, seconds=12.0, sample_s=10.0):
+    """The oracle's C restatement of the recursion (oracle/c/mpf_c.c through oracle/c_chain.py: the level loop
+    of Cauchy and pole steps, bit-identical to the GPU kernels' spec) on every host thread of this process.
+
+    Measured: one whole recursion of a real operator group of the workload at its real shape (order 60,
+    2 j_max 60, 1024 bits, the 11 structures of one sign as right-hand sides), the largest group whose
+    predicted time fits sample_s.  The other groups are priced by a per-unit model (Cauchy entry, Cauchy term,
+    pole dot product, each timed on the same threads) calibrated by that run.  Derivative, output and text
     stages are not counted, so the figure is an upper bound on the CPU rate."""
+    import numpy as np
+    from oracle import c_chain as CC
     from oracle import c_ref as C
     L = 34
     th = max(1, len(os.sched_getaffinity(0)))
@@ -316,15 +332,26 @@ def cpu_baseline(groups, seconds=12.0):
     b = (t4 - t1) / 3
     a = t1 - b
     tdot = {k: C.time_unit(1, L, k, th, part) for k in range(1, 6)}
-    tot = 0.0
-    for g in groups:
-        pl = g.plan
-        rhs = sum(NQ_PER_SIGN * len(bt.ns) for bt in g.batches)
+
+    def per_rhs(pl):
         entries = sum(lv.nH for lv in pl.levels if lv) * pl.S2
         terms = pl.cauchy_terms() * pl.S2
         pole = sum(lv.T for lv in pl.levels if lv) * pl.S2 * (tdot[min(5, pl.SRm)] + tdot[min(5, pl.SLm)])
-        tot += rhs * (entries * a + terms * b + pole)
-    blocks = sum(bt.blocks() for g in groups for bt in g.batches)
+        return entries * a + terms * b + pole
+
+    fits = [g for g in groups if per_rhs(g.plan) * NQ_PER_SIGN <= sample_s]
+    g = max(fits, key=lambda x: per_rhs(x.plan)) if fits else min(groups, key=lambda x: per_rhs(x.plan))
+    pl = g.plan
+    rng = np.random.default_rng(5)
+    Mtab = _random_pairs(rng, max(len(pl.Mkeys), 1) * pl.Smax * pl.Smax, L, 1)
+    hm, he = CC.planes(_random_pairs(rng, 2 * (ORDER + 1) * pl.Jh * pl.S2 * NQ_PER_SIGN, L, 2), L)
+    tm = {}
+    CC.recursion(pl, Mtab, hm, he, L, NQ_PER_SIGN, threads=th, timing=tm)
+    measured = tm["seconds"]
+    ratio = measured / (per_rhs(pl) * NQ_PER_SIGN)
+    tot = ratio * sum(per_rhs(x.plan) * NQ_PER_SIGN * len(bt.ns) for x in groups for bt in x.batches)
+    blocks = sum(bt.blocks() for x in groups for bt in x.batches)
+    sample_blocks = g.batches[0].blocks() / (measured * len(g.batches[0].ns))
     model = ""
     try:
         for line in open("/proc/cpuinfo"):
@@ -334,10 +361,14 @@ def cpu_baseline(groups, seconds=12.0):
     except OSError:
         pass
     return {"value": blocks / tot, "unit": "blocks/s", "cores": th, "kind": "port", "cpu_model": model,
-            "sample": f"oracle/c/mpf_c.c on {th} threads for {seconds:.0f} s: Cauchy entry {a * 1e9:.0f} ns + "
-                      f"{b * 1e9:.0f} ns per term, 5-term pole dot {tdot[5] * 1e9:.0f} ns (aggregate per unit), "
-                      f"scaled by the workload's recursion entry/term counts; derivative, output and text stages "
-                      f"not counted (blocks_3d itself is not on this machine)"}
+            "measured_group": f"{g.tj12},{g.tj43}", "measured_recursion_s": measured,
+            "measured_group_blocks_per_s": sample_blocks, "model_calibration": ratio,
+            "sample": f"oracle/c/mpf_c.c level loop (oracle/c_chain.py) on {th} threads: one whole recursion of "
+                      f"operator group ({g.tj12},{g.tj43}) at order {ORDER}, 1024 bits, {NQ_PER_SIGN} right-hand "
+                      f"sides timed ({measured:.1f} s); the other groups priced by per-unit costs (Cauchy entry "
+                      f"{a * 1e9:.0f} ns + {b * 1e9:.0f} ns per term, 5-term pole dot {tdot[5] * 1e9:.0f} ns, "
+                      f"{seconds:.0f} s of timing) scaled by that run ({ratio:.2f}x); derivative, output and text "
+                      f"stages not counted (blocks_3d itself is not on this machine)"}
 
 
 def emission_check(batch, emitter_cls, outdir):

@@ -24,10 +24,11 @@ _TABLES = {}
 _TABLE_LOCK = threading.Lock()
 WORKSPACE_BYTES = 1 << 31
 
-# MFMA block tiles of b3d_rns_gemm (16 wm rows x 64 / wm columns).  With BLOCKSVIBE_RNS_TUNE=on the first GEMM
-# of a shape class times every candidate (they compute the same bits) and keeps the fastest for the rest of the
-# process; BLOCKSVIBE_RNS_TUNE=<file> also keeps the choices in that JSON file across processes (and only
-# tunes classes it does not hold).  Unset: the tiles of the file BLOCKSVIBE_RNS_TILES names if any, else wm = 1.
+# MFMA block tiles of b3d_rns_gemm (16 wm rows x 64 / wm columns).  By default (BLOCKSVIBE_RNS_TUNE unset or
+# "on") the first GEMM of a shape class times every candidate (they compute the same bits) and keeps the fastest
+# for the rest of the process; BLOCKSVIBE_RNS_TUNE=<file> also keeps the choices in that JSON file across
+# processes (and only tunes classes it does not hold).  BLOCKSVIBE_RNS_TUNE=off: the tiles of the file
+# BLOCKSVIBE_RNS_TILES names if any, else wm = 1.
 TILES = (1, 2, 4)
 _TUNED = {}
 _TUNE_LOCK = threading.Lock()
@@ -35,7 +36,7 @@ _TUNE_LOADED = [False]
 
 
 def _tune_mode():
-    return os.environ.get("BLOCKSVIBE_RNS_TUNE", "")
+    return os.environ.get("BLOCKSVIBE_RNS_TUNE", "") or "on"
 
 
 def _tune_file():
@@ -68,7 +69,7 @@ def _load_tuning():
 
 def _save_tuning():
     f = _tune_file()
-    if not f or _tune_mode() in ("", "off"):
+    if not f or _tune_mode() == "off":
         return
     tmp = f + ".tmp"
     with open(tmp, "w") as fh:
@@ -82,7 +83,7 @@ def _run_tuned(key, run):
     with _TUNE_LOCK:
         _load_tuning()
         wm = _TUNED.get(key)
-    if wm is not None or _tune_mode() in ("", "off") or torch.cuda.is_current_stream_capturing():
+    if wm is not None or _tune_mode() == "off" or torch.cuda.is_current_stream_capturing():
         run(wm or 1)
         return
     times = {}
