@@ -46,7 +46,7 @@ __device__ __forceinline__ unsigned b3d_xcd_block(unsigned b, unsigned G) {
 // Limb counts with compiled kernels.  L = 2 ceil(precision / 64) + 2 (precision rounded up to whole
 // 64-bit limbs, plus >= 60 guard bits): every precision that is a multiple of 128 bits up to 1536 bits,
 // plus the multiples of 64 bits the configured workloads use.
-#define B3D_FOR_EACH_L(X) X(4) X(6) X(10) X(12) X(14) X(18) X(22) X(26) X(30) X(34) X(38) X(42) X(46) X(50)
+#define B3D_FOR_EACH_L(X) X(4) X(6) X(10) X(12) X(14) X(18) X(22) X(26) X(30) X(34) X(38) X(42) X(46) X(50) X(54)
 
 #define B3D_CASE_L(Lv, ...) \
   case Lv: {                \
@@ -71,9 +71,10 @@ __device__ __forceinline__ unsigned b3d_xcd_block(unsigned b, unsigned G) {
     B3D_CASE_L(42, __VA_ARGS__)                                                          \
     B3D_CASE_L(46, __VA_ARGS__)                                                          \
     B3D_CASE_L(50, __VA_ARGS__)                                                          \
+    B3D_CASE_L(54, __VA_ARGS__)                                                          \
     default:                                                                      \
       b3d::set_error("unsupported limb count L=" + std::to_string(L) +            \
-                     " (compiled: 4 6 10 12 14 18 22 26 30 34 38 42 46 50)");     \
+                     " (compiled: 4 6 10 12 14 18 22 26 30 34 38 42 46 50 54)");     \
       return 2;                                                                   \
   }
 

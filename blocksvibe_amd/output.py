@@ -3,8 +3,8 @@ in the xt or xt_radial coordinates (reference blocksvibe/output.py:1-212; blocks
 lambda_lr_to_lrho_lrhob, congruence_transform, zzb_to_xt, lrho_to_xt_radial).
 
 Every step is a scalar linear map acting on each x^k coefficient separately, so the whole chain is one
-matrix W (outputs x inputs) per (coordinate, Lambda, parity, Delta_1+2), built once on the host in mpmath at
-32 L + 128 bits and applied to every spin, structure, coefficient and task as one RNS GEMM:
+matrix W (outputs x inputs) per (coordinate, Lambda, parity, Delta_1+2), built once and applied to every
+spin, structure, coefficient and task as one RNS GEMM:
   xt_radial: W = pre R_z S  (order O = Lambda - parity; pre = power_prefactor(O, D) for parity 0,
              one_minus_power_prefactor(O) power_prefactor(O, D + 1) for parity 1);
   xt:        F_u = the (lambda, log r) -> (lambda_rho, lambda_rhobar) image of input u = (n_lambda index, e)
@@ -12,6 +12,14 @@ matrix W (outputs x inputs) per (coordinate, Lambda, parity, Delta_1+2), built o
              Delta_1+2 != 0), W[(n, b), u] = ((m - p)/2)! / m! sum_{k, l} (-1)^l C(n, k) C(m, l)
              Z_u(n + m - k - l, k + l), m = p + 2 b.
 S is the Stirling-number matrix scaled by r_c^-n, R_z the inverse of blocks_3d's rho -> z series matrix.
+
+Construction (build_map_device): the (O + 1)^2 matrices S, R_z, P and their product M on the host in mpmath
+(O^3 operations), then W on the GPU at LG >= L + 4 limbs (128 guard bits, as the host construction's
+32 L + 128): for each antidiagonal s, T_s[(a, b), k] = M[a, k] M[b, s - k] (elementwise products over the
+pairs a + b <= O the map reads) and Z[(a, b), u] = T_s G_s with G_s[k, u] = F_u(k, s - k) (exact); then
+W = C Z with C[(n, b), (a, b')] = ((m - p)/2)! / m! sum_l (-1)^l C(n, b' - l) C(m, l) on a + b' = n + m
+(exact), both RNS GEMMs; W is rounded once more to L limbs.  build_map is the same map entirely in mpmath
+(O^5 operations at O = Lambda: minutes at Lambda = 43), kept to check the device construction.
 """
 import math
 
@@ -141,6 +149,95 @@ def build_map(coordinate, Lam, parity, delta_1_plus_2, L):
     return W, inputs, outputs
 
 
+def guard_limbs(L):
+    """the limb count of the map construction: the smallest compiled one >= L + 4"""
+    from . import _lib
+    for LG in range(L + 4, L + 17):
+        if _lib.lib().b3d_supported_limbs(LG):
+            return LG
+    raise ValueError(f"no compiled limb count in [{L + 4}, {L + 16}] for the map construction at L = {L}")
+
+
+def _xt_factors(O, p):
+    """host exact parts of the xt map: inputs, outputs, the per-input antidiagonal images F_u(k, s - k) as
+    Fractions, and the output rows' coefficients on the Z pairs (a, b) (a + b = n + m) as Fractions"""
+    from fractions import Fraction
+    sig = -1 if p % 2 else 1
+    inputs = [(idx, e) for idx in range((O - p) // 2 + 1) for e in range(O - (p + 2 * idx) + 1)]
+    outputs = [(n, b) for n in range(O - p + 1) for b in range((O - n - p) // 2 + 1)]
+    G = []
+    for idx, e in inputs:
+        s = p + 2 * idx + e
+        col = []
+        for i in range(s + 1):
+            j = s - i
+            hi, lo = (i, j) if i >= j else (j, i)
+            if lo > min(hi, O - hi):
+                col.append(Fraction(0))
+                continue
+            acc = 0
+            for a in range(max(0, e - lo), min(hi, e) + 1):
+                acc += (-1) ** (lo - (e - a)) * math.comb(hi, a) * math.comb(lo, e - a)
+            v = Fraction(acc, 1 << s)
+            col.append(v if i >= j else sig * v)
+        G.append((s, col))
+    C = []
+    for n, b in outputs:
+        m = p + 2 * b
+        fac = Fraction(math.factorial((m - p) // 2), math.factorial(m))
+        row = {}
+        for jj in range(n + m + 1):
+            c = sum((-1) ** l * math.comb(n, jj - l) * math.comb(m, l) for l in range(max(0, jj - n), min(m, jj) + 1))
+            if c:
+                row[(n + m - jj, jj)] = fac * c
+        C.append(row)
+    return inputs, outputs, G, C
+
+
+def build_map_device(coordinate, Lam, parity, delta_1_plus_2, L):
+    """(W as a device MPF (outputs, inputs) at L limbs, inputs [(idx, e)], outputs [(n, b)]): the map of
+    build_map, its O^5 part on the GPU (module docstring)"""
+    p = parity
+    if coordinate == "xt_radial":
+        W, inputs, outputs = build_map(coordinate, Lam, parity, delta_1_plus_2, L)   # O^3 on the host
+        return mpf.from_mp([x for row in W for x in row], L, (len(outputs), len(inputs))), inputs, outputs
+    if coordinate != "xt":
+        raise ValueError(coordinate)
+    LG = guard_limbs(L)
+    O = Lam
+    mp.mp.prec = 32 * LG + 64
+    D = mp.mpf(delta_1_plus_2)
+    M = rho_to_z(O) * stirling_matrix(O)
+    if D != 0:
+        M = power_prefactor(O, D / 2) * M
+    inputs, outputs, G, C = _xt_factors(O, p)
+    O1 = O + 1
+    Md = mpf.from_mp([M[i, j] for i in range(O1) for j in range(O1)], LG, (O1 * O1,))
+    pairs = [(a, b) for a in range(O1) for b in range(O1 - a)]
+    pidx = {ab: i for i, ab in enumerate(pairs)}
+    dev = Md.e.device
+    pa = torch.tensor([a for a, _ in pairs], dtype=torch.int64, device=dev)
+    pb = torch.tensor([b for _, b in pairs], dtype=torch.int64, device=dev)
+    Z = mpf.zeros((len(pairs), len(inputs)), LG)
+    by_s = {}
+    for u, (s, _) in enumerate(G):
+        by_s.setdefault(s, []).append(u)
+    for s, us in sorted(by_s.items()):
+        k = torch.arange(s + 1, dtype=torch.int64, device=dev)
+        Ta = Md.take((pa[:, None] * O1 + k[None, :]).reshape(-1))
+        Tb = Md.take((pb[:, None] * O1 + (s - k)[None, :]).reshape(-1))
+        T = mpf.mul(Ta, Tb).reshape(len(pairs), s + 1)
+        Gs = mpf.from_fractions([G[u][1][kk] for kk in range(s + 1) for u in us], LG, (s + 1, len(us)))
+        Zs = rns.gemm(T, Gs)
+        cols = torch.tensor(us, dtype=torch.int64, device=dev)
+        Z.m[:, :, cols] = Zs.m
+        Z.e[:, cols] = Zs.e
+    Cd = mpf.from_fractions([row.get(ab, 0) for row in C for ab in pairs], LG, (len(outputs), len(pairs)))
+    Wg = rns.gemm(Cd, Z)
+    W = mpf.norm_wide(Wg.m.reshape(LG, -1).contiguous(), Wg.e.reshape(-1).contiguous(), L)
+    return W.reshape(len(outputs), len(inputs)), inputs, outputs
+
+
 class Output:
     """write_output for one coordinate ("xt" | "xt_radial"), Lambda, parity (= (1 - four_pt_sign) / 2) and
     Delta_1+2 (decimal string) at L limbs."""
@@ -148,8 +245,7 @@ class Output:
     def __init__(self, coordinate, Lam, parity, delta_1_plus_2, L, W=None, inputs=None, outputs=None):
         self.coord, self.Lam, self.parity, self.d12, self.L = coordinate, Lam, parity, delta_1_plus_2, L
         if W is None:
-            Wl, inputs, outputs = build_map(coordinate, Lam, parity, delta_1_plus_2, L)
-            W = mpf.from_mp([x for row in Wl for x in row], L, (len(outputs), len(inputs)))
+            W, inputs, outputs = build_map_device(coordinate, Lam, parity, delta_1_plus_2, L)
         self.W, self.inputs, self.outputs = W, list(inputs), list(outputs)
         self.nL = (Lam - parity) // 2 + 1
         # rows of each lambda order's derivatives that the map reads (device index tables, built once)
@@ -188,7 +284,7 @@ def cached(coordinate, Lam, parity, delta_1_plus_2, L, cache_dir=None):
     import os
     if not cache_dir:
         return Output(coordinate, Lam, parity, delta_1_plus_2, L)
-    tag = hashlib.sha1(f"{coordinate}|{Lam}|{parity}|{delta_1_plus_2}|{L}|map2".encode()).hexdigest()[:20]
+    tag = hashlib.sha1(f"{coordinate}|{Lam}|{parity}|{delta_1_plus_2}|{L}|map3".encode()).hexdigest()[:20]
     path = os.path.join(cache_dir, f"wmap_{coordinate}_L{Lam}_p{parity}_{tag}.pt")
     try:
         st = torch.load(path, weights_only=True)
