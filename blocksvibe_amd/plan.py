@@ -203,43 +203,50 @@ class Plan:
         self.dmax = max(lev.dmax for lev in lv)
 
     def _physical(self, reuse):
+        """physical slots: without reuse one per written logical slot; with reuse a logical slot's physical slot
+        goes back to a LIFO stack once its last reader's level has run (the level after its own if nothing
+        reads it), and the slots born at a level take the most recently returned physical slots first, in
+        logical order, before new ones are numbered.  Returns wait in per-level lists that are handed back in
+        the order their return level was first scheduled (a skipped level hands back several at once), so the
+        numbering, and with it the published slot counts, is reproduced exactly."""
         N = 2 * self.nslots
-        born = np.full(N, -1, dtype=np.int64)
-        last = np.full(N, -1, dtype=np.int64)
+        written = np.full(N, -1, dtype=np.int64)           # level that writes the slot
+        read_to = np.full(N, -1, dtype=np.int64)           # last level that reads it
         for lev in self.levels:
             if lev is None:
                 continue
-            born[lev.t_slot] = lev.m
-            c = lev.col.reshape(-1)
-            np.maximum.at(last, c[c >= 0], lev.m)
+            written[lev.t_slot] = lev.m
+            cols = lev.col.reshape(-1)
+            np.maximum.at(read_to, cols[cols >= 0], lev.m)
         phys = np.full(N, -1, dtype=np.int64)
         if not reuse:
-            phys[born >= 0] = np.flatnonzero(born >= 0)
+            live = np.flatnonzero(written >= 0)
+            phys[live] = live
             self.nphys = N
         else:
-            free = np.zeros(0, dtype=np.int64)
-            release = {}                                           # level -> list of arrays (insertion order)
-            nxt = 0
+            stack = np.zeros(0, dtype=np.int64)              # free physical slots, top at the end
+            returns = {}                                     # return level -> [physical slot arrays]
+            made = 0
             for lev in self.levels:
                 if lev is None:
                     continue
-                m = lev.m
-                seg = np.sort(lev.t_slot)                          # born at m, in logical order
-                for kk in [kk for kk in release if kk <= m]:
-                    free = np.concatenate([free] + release.pop(kk))
-                take = min(len(seg), len(free))
-                ph = np.empty(len(seg), dtype=np.int64)
-                ph[:take] = free[::-1][:take]                      # LIFO
-                ph[take:] = nxt + np.arange(len(seg) - take)
-                nxt += len(seg) - take
-                free = free[:len(free) - take]
-                phys[seg] = ph
-                d = last[seg]
-                rel = np.where(d > m, d, m + 1)
-                _, fi = np.unique(rel, return_index=True)
-                for kk in rel[np.sort(fi)]:
-                    release.setdefault(int(kk), []).append(ph[rel == kk])
-            self.nphys = nxt
+                here = lev.m
+                due = [r for r in returns if r <= here]      # scheduling order, not numeric order
+                if due:
+                    stack = np.concatenate([stack] + [a for r in due for a in returns.pop(r)])
+                born = np.sort(lev.t_slot)
+                k = min(len(born), len(stack))
+                got = np.empty(len(born), dtype=np.int64)
+                got[:k] = stack[len(stack) - k:][::-1]
+                got[k:] = np.arange(made, made + len(born) - k)
+                made += len(born) - k
+                stack = stack[:len(stack) - k]
+                phys[born] = got
+                back_at = np.maximum(read_to[born], here + 1)
+                levels_in_order = back_at[np.sort(np.unique(back_at, return_index=True)[1])]
+                for r in levels_in_order:
+                    returns.setdefault(int(r), []).append(got[back_at == r])
+            self.nphys = made
         self.phys = phys
         self.zero_slot = self.nphys
         for lev in self.levels:
