@@ -11,7 +11,8 @@ tests/test_blocks_recursion.py:12-52, tests/test_dropin.py:20-59):
         2^-(512 - 32)
   cfg3  the heaviest operator group of the nmax = 6 workload ((2j12, 2j43) = (8, 8), spins 0..30, order 60,
         1024 bits), two right-hand sides: every residue slot bit for bit against the C restatement of the
-        recursion (oracle/c_chain.py), and sampled numerators (kept 14) within 2^-(1024 - 32) of oracle.derivs_ref
+        recursion (oracle/c_chain.py; 25 of the 60 levels take k_cauchy_pairs, the others k_cauchy_groups, and
+        the pole step is the VALU path on 5 x 5 blocks), and sampled numerators (kept 14) within 2^-(1024 - 32) of oracle.derivs_ref
         relative to each polynomial's largest coefficient
   cfg4  a serving mix of limb counts 10..42 and nmax 2..8 with both signs: each request's result identical alone
         and in the mix
@@ -192,6 +193,9 @@ def test_cfg3_heavy_group_bit_exact():
     timing = {}
     cm, ce = CC.recursion(plan, Mtab, hm, he, L, B, threads=threads, timing=timing)
     print(f"C recursion: {timing['seconds']:.1f} s", flush=True)
+    # both Cauchy kernels met the oracle: two rows per lane on the levels with enough waves, one on the others
+    rpl = [w[4] for w in rec.wave_tables(plan.S2 * B) if w is not None]
+    assert 1 in rpl and 2 in rpl, rpl
     zero = ge == R.EZ
     assert np.array_equal(zero, ce == R.EZ)
     assert np.array_equal(ge[~zero], ce[~zero])
