@@ -86,6 +86,15 @@ def zeros(shape, L, device=None):
                torch.full(tuple(shape), EZ, dtype=torch.int64, device=device))
 
 
+def empty(shape, L, device=None):
+    """uninitialised MPF, for outputs that a kernel writes in full before anything reads them"""
+    device = device or dev()
+    if isinstance(shape, int):
+        shape = (shape,)
+    return MPF(torch.empty((L,) + tuple(shape), dtype=torch.int32, device=device),
+               torch.empty(tuple(shape), dtype=torch.int64, device=device))
+
+
 def cat(parts, dim):
     return MPF(torch.cat([p.m for p in parts], dim + 1), torch.cat([p.e for p in parts], dim))
 
@@ -198,7 +207,7 @@ def _ptr(t):
 def add(a, b, out=None):
     """elementwise a + b (b may have a single entry: broadcast)"""
     L, n = a.L, a.numel()
-    out = out or zeros(a.shape, L, a.e.device)
+    out = out or empty(a.shape, L, a.e.device)
     a, b = a.contiguous(), b.contiguous()
     sb = 0 if b.numel() == 1 else 1
     _lib.call("b3d_mpf_add", L, _ptr(a.m), _ptr(a.e), a.numel(), 1, _ptr(b.m), _ptr(b.e), b.numel(), sb,
@@ -209,7 +218,7 @@ def add(a, b, out=None):
 def mul(a, b, out=None):
     """elementwise a * b (b may have a single entry: broadcast)"""
     L, n = a.L, a.numel()
-    out = out or zeros(a.shape, L, a.e.device)
+    out = out or empty(a.shape, L, a.e.device)
     a, b = a.contiguous(), b.contiguous()
     sb = 0 if b.numel() == 1 else 1
     _lib.call("b3d_mpf_mul", L, _ptr(a.m), _ptr(a.e), a.numel(), 1, _ptr(b.m), _ptr(b.e), b.numel(), sb,
@@ -232,7 +241,7 @@ def align_rows(a):
 def norm_wide(W, ein, L):
     """W: int32 tensor (LW, n) of wide two's complement integers, ein: int64 (n) -> normalised MPF (n)"""
     LW, n = W.shape[0], W.shape[1]
-    out = zeros((n,), L, W.device)
+    out = empty((n,), L, W.device)
     _lib.call("b3d_mpf_norm_wide", L, LW, _ptr(W), n, _ptr(ein), _ptr(out.m), _ptr(out.e), n, n, stream_ptr())
     return out
 

@@ -73,6 +73,7 @@ class Plan:
         self._first_flat = np.concatenate(self.level_first).astype(np.int64) if order else np.zeros(0, np.int64)
         self.levels = [self._level(m) for m in range(order)]
         self._keys()
+        self.reuse = reuse
         self._physical(reuse)
 
     # ------------------------------------------------------------------------------------------------

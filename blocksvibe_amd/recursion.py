@@ -213,13 +213,20 @@ class Recursion:
         B = h_inf.shape[-1]
         N = S2 * B
         h = h_inf.contiguous()
+        # With slot recycling every physical slot is written in full (by the pole step of its level) before
+        # anything reads it, so only the zero slot needs clearing; without it, logical slots that are never
+        # written keep their place in the buffer and are cleared with the rest.  H and Y are written in full
+        # for the rows a level reads.
         if res is None:
-            res = mpf.zeros((plan.nphys + 1, S2, B), L)
-        else:
+            res = (mpf.empty if plan.reuse else mpf.zeros)((plan.nphys + 1, S2, B), L)
+        elif not plan.reuse:
             res.m.zero_()
             res.e.fill_(mpf.EZ)
-        H = mpf.zeros((self.maxH, S2, B), L)
-        Y = mpf.zeros((self.maxT, SR, SL, B), L)
+        if plan.reuse:
+            res.m[:, plan.nphys].zero_()
+            res.e[plan.nphys].fill_(mpf.EZ)
+        H = mpf.empty((self.maxH, S2, B), L)
+        Y = mpf.empty((self.maxT, SR, SL, B), L)
         st = mpf.stream_ptr()
         P = lambda t: t.data_ptr()
         waves = self.wave_tables(N) if self.group_waves else [(None, None, None, 0, 1)] * len(self.levels)

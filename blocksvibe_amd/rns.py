@@ -281,7 +281,7 @@ def gemm(A, B, out=None, budget=None, tile=None):
         Mr, K = A.shape
         K2, N = B.shape
         assert K == K2
-        C = out or mpf.zeros((Mr, N), L, A.e.device)
+        C = out or (mpf.empty if K else mpf.zeros)((Mr, N), L, A.e.device)   # written in full when K > 0
         return gemm_strided(L, 1, Mr, N, K, A, (0, K, 1), True, B, (0, N, 1), C, (0, N, 1), budget, tile)
     nb, K, N = B.shape
     if len(A.shape) == 2:
@@ -292,5 +292,5 @@ def gemm(A, B, out=None, budget=None, tile=None):
         assert nbA == nb
         shared, asb = False, Mr * K2
     assert K == K2, (A.shape, B.shape)
-    C = out or mpf.zeros((nb, Mr, N), L, A.e.device)
+    C = out or (mpf.empty if K else mpf.zeros)((nb, Mr, N), L, A.e.device)
     return gemm_strided(L, nb, Mr, N, K, A, (asb, K, 1), shared, B, (K * N, N, 1), C, (Mr * N, N, 1), budget, tile)
