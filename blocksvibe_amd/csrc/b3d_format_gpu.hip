@@ -97,11 +97,24 @@ __global__ __launch_bounds__(256) void k_fmt_text(const int32_t* __restrict__ m,
   constexpr int FW = Fmt<L>::FW;
   for (i64 k = blockIdx.x * (i64)blockDim.x + threadIdx.x; k < n; k += (i64)gridDim.x * blockDim.x) {
     const i64 i = ent[k];
-    auto put = [&](int c, char v) { tx[(i64)c * n + k] = (uint8_t)v; };
-    auto dput = [&](int c, int v) { dg[(i64)c * n + k] = (uint8_t)v; };
-    auto dget = [&](int c) -> int { return dg[(i64)c * n + k]; };
+    // text and digit staging in groups of four bytes: byte c of element k at word (c / 4) * n + k, byte c % 4,
+    // so a wave's accesses to one group cover 256 contiguous bytes.  The text is collected a word at a time
+    // (its bytes are produced in order) and stored whole.
+    u32* const txw = (u32*)tx;
+    u32 tw = 0u;
+    auto put = [&](int c, char v) {
+      tw |= (u32)(uint8_t)v << (8 * (c & 3));
+      if ((c & 3) == 3) {
+        txw[(i64)(c >> 2) * n + k] = tw;
+        tw = 0u;
+      }
+    };
+    auto dat = [&](int c) -> i64 { return (((i64)(c >> 2) * n + k) << 2) + (c & 3); };
+    auto dput = [&](int c, int v) { dg[dat(c)] = (uint8_t)v; };
+    auto dget = [&](int c) -> int { return dg[dat(c)]; };
+    auto dword = [&](int g) -> u32 { return ((const u32*)dg)[(i64)g * n + k]; };
     if (i < 0 || e[i] == B3D_EZ) {   // the "0" of an all-zero polynomial, or a zero coefficient
-      put(0, '0');
+      txw[k] = (u32)'0';
       len[k] = 1;
       continue;
     }
@@ -206,12 +219,21 @@ __global__ __launch_bounds__(256) void k_fmt_text(const int32_t* __restrict__ m,
     while (nd > 1 && dget(nd - 1) == 0) --nd;
     // layout (C's %g): d.ddde+XX outside -4 <= X < p, fixed notation inside
     int c = 0;
+    // digits q0 .. q1 - 1 as text, read a staged word (four digits) at a time
+    auto digits = [&](int q0, int q1) {
+      int q = q0;
+      while (q < q1) {
+        const u32 w = dword(q >> 2);
+        const int qe = q1 < ((q | 3) + 1) ? q1 : ((q | 3) + 1);
+        for (; q < qe; ++q) put(c++, (char)('0' + ((w >> (8 * (q & 3))) & 0xFFu)));
+      }
+    };
     if (neg) put(c++, '-');
     if (X < -4 || X >= p) {
       put(c++, (char)('0' + dget(0)));
       if (nd > 1) {
         put(c++, '.');
-        for (int q = 1; q < nd; ++q) put(c++, (char)('0' + dget(q)));
+        digits(1, nd);
       }
       put(c++, 'e');
       put(c++, X < 0 ? '-' : '+');
@@ -225,19 +247,36 @@ __global__ __launch_bounds__(256) void k_fmt_text(const int32_t* __restrict__ m,
       if (w < 2) tmp[w++] = '0';
       while (w) put(c++, tmp[--w]);
     } else if (X >= 0) {
-      for (int q = 0; q <= X; ++q) put(c++, q < nd ? (char)('0' + dget(q)) : '0');
+      digits(0, nd < X + 1 ? nd : X + 1);
+      for (int q = nd; q <= X; ++q) put(c++, '0');
       if (nd > X + 1) {
         put(c++, '.');
-        for (int q = X + 1; q < nd; ++q) put(c++, (char)('0' + dget(q)));
+        digits(X + 1, nd);
       }
     } else {
       put(c++, '0');
       put(c++, '.');
       for (int q = 0; q < -1 - X; ++q) put(c++, '0');
-      for (int q = 0; q < nd; ++q) put(c++, (char)('0' + dget(q)));
+      digits(0, nd);
     }
+    if (c & 3) txw[(i64)(c >> 2) * n + k] = tw;   // the last, partial word
     len[k] = c;
   }
+}
+
+// the text of element k (l bytes, in groups of four: word (c / 4) * n + k) to o: four words (16 bytes) loaded
+// per step, then stored byte by byte (o has any alignment); returns o + l
+__device__ __forceinline__ uint8_t* copy_text(const uint8_t* __restrict__ tx, i64 n, i64 k, int l, uint8_t* o) {
+  const u32* src = (const u32*)tx + k;
+  for (int g0 = 0; 4 * g0 < l; g0 += 4) {
+    u32 w[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) w[j] = 4 * (g0 + j) < l ? src[(i64)(g0 + j) * n] : 0u;
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+      if (4 * g0 + j < l) o[4 * g0 + j] = (uint8_t)(w[j >> 2] >> (8 * (j & 3)));
+  }
+  return o + l;
 }
 
 __global__ __launch_bounds__(256) void k_fmt_pack(const uint8_t* __restrict__ tx, const int32_t* __restrict__ len,
@@ -252,7 +291,7 @@ __global__ __launch_bounds__(256) void k_fmt_pack(const uint8_t* __restrict__ tx
       for (i64 c = stroff[pre[k]]; c < stroff[pre[k] + 1]; ++c) *o++ = strtab[c];
     const int l = len[k];
     *o++ = '"';
-    for (int c = 0; c < l; ++c) *o++ = tx[(i64)c * n + k];
+    o = copy_text(tx, n, k, l, o);
     *o++ = '"';
     if (!last[k]) {
       *o++ = ',';
@@ -285,14 +324,14 @@ __device__ __forceinline__ void write_piece(const uint8_t* __restrict__ tx, cons
     for (i64 c = stroff[pre[k]]; c < stroff[pre[k] + 1]; ++c) *o++ = strtab[c];
   const int l = len[k];
   *o++ = '"';
-  for (int c = 0; c < l; ++c) *o++ = tx[(i64)c * n + k];
+  o = copy_text(tx, n, k, l, o);
   *o++ = '"';
   if (!last[k]) {
     *o++ = ',';
     *o++ = '\n';
   }
   if (post[k] >= 0)
-    for (i64 c = stroff[post[k]]; c < stroff[post[k] + 1]; ++c) *o++ = strtab[c];
+    for (i64 c2 = stroff[post[k]]; c2 < stroff[post[k] + 1]; ++c2) *o++ = strtab[c2];
 }
 
 __global__ __launch_bounds__(128) void k_fmt_pack_lds(const uint8_t* __restrict__ tx, const int32_t* __restrict__ len,

@@ -352,8 +352,8 @@ class Emitter:
         pre_el = pre_el.reshape(-1)
         post_el = torch.where(lastf, tpost[:, None], torch.full((1, 1), -1, dtype=torch.int32, device=dev))
         post_el = post_el.reshape(-1).contiguous()
-        dg = torch.empty(n * (p + 10 * L + 32), dtype=torch.uint8, device=dev)
-        tx = torch.empty(n * (p + 16), dtype=torch.uint8, device=dev)
+        dg = torch.empty(n * ((p + 10 * L + 32 + 3) // 4 * 4), dtype=torch.uint8, device=dev)
+        tx = torch.empty(n * ((p + 16 + 3) // 4 * 4), dtype=torch.uint8, device=dev)   # groups of four bytes
         ln = torch.empty(n, dtype=torch.int32, device=dev)
         _lib.call("b3d_fmt_text", L, P(m2), P(e1), ps, P(ent), n, p, P(dg), P(tx), P(ln), sptr)
         del dg

@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-#define B3D_ABI_VERSION 13
+#define B3D_ABI_VERSION 14
 
 /* Last error message of the calling thread (empty if none). */
 const char* b3d_last_error(void);
@@ -222,9 +222,10 @@ int b3d_format_polys(int L, const int32_t* m, const int64_t* e, int64_t ps, int6
 int b3d_fmt_sizes(int L, const int32_t* m, const int64_t* e, int64_t ps, const int64_t* pbase, int64_t npoly, int deg,
                   int64_t eps_exp, int32_t* sizes, void* stream);
 /* Text of n elements: element k is entry ent[k] of the planes (ent[k] < 0: the text "0"), formatted as
- * b3d_format_numbers does, byte c of its text at text[c * n + k] (text: n * (p + 16) bytes), its length at
- * len[k]; len[k] = -1 for a value outside the GPU's range (exponent >= 0, or fraction bits beyond
- * 32 (L + 16)), to be formatted on the host.  digits: staging of n * (p + 10 L + 32) bytes. */
+ * b3d_format_numbers does, in groups of four bytes: byte c of its text at text[((c / 4) * n + k) * 4 + c % 4]
+ * (text: n * R(p + 16) bytes, R(x) = x rounded up to a multiple of 4), its length at len[k]; len[k] = -1 for a
+ * value outside the GPU's range (exponent >= 0, or fraction bits beyond 32 (L + 16)), to be formatted on the
+ * host.  digits: staging of n * R(p + 10 L + 32) bytes.  text and digits 4-byte aligned. */
 int b3d_fmt_text(int L, const int32_t* m, const int64_t* e, int64_t ps, const int64_t* ent, int64_t n, int p,
                  uint8_t* digits, uint8_t* text, int32_t* len, void* stream);
 /* Packed bodies: element k as '"' text '"' followed by ",\n" unless last[k], at out + off[k]. */
