@@ -206,42 +206,36 @@ def make_bins(batches, nbins):
     return [x for x in bins if x]
 
 
-def lanes(batches, n):
-    """the batches of one step dealt into n lanes of about equal cost (largest first, each to the cheapest lane
-    so far).  Lanes may share an operator group: its constants are read-only once built, and those built on
-    first use are published only after their stream has finished them"""
-    out, load = [[] for _ in range(n)], [0.0] * n
-    for b in sorted(batches, key=lambda x: -x.cost()):
-        k = min(range(n), key=lambda j: load[j])
-        out[k].append(b)
-        load[k] += b.cost()
-    return [x for x in out if x]
-
-
 _STREAMS = []
 
 
 def step(bins, i, emitter, timer=None, streams=1):
     """one step: the chain and the spin files of bin i % len(bins); returns its blocks.  streams > 1: the
-    step's batches in that many lanes, each lane on its own HIP stream driven by its own thread, so that one
-    lane's small levels share the GPU with another lane's work"""
+    step's batches run in that many lanes, each lane on its own HIP stream driven by its own thread, so that one
+    lane's small levels and file emission share the GPU with another lane's chain"""
+    import queue
     import threading
     import torch
     batches = bins[i % len(bins)]
-    ls = lanes(batches, streams) if streams > 1 else [batches]
+    nl = min(streams, len(batches))
 
-    def run_lane(lane):
-        for b in lane:
-            r = b.run(timer=timer)
-            if emitter is not None:
-                b.emit(emitter, r)
-            del r
+    def run_one(b):
+        r = b.run(timer=timer)
+        if emitter is not None:
+            b.emit(emitter, r)
+        del r
 
-    if len(ls) == 1:
-        run_lane(ls[0])
+    if nl <= 1:
+        for b in batches:
+            run_one(b)
     else:
-        while len(_STREAMS) < len(ls):
+        # the lanes take the step's batches from one queue, largest first, so that both streams stay busy
+        # until the last batch has started (a fixed split leaves one idle when the costs are uneven)
+        while len(_STREAMS) < nl:
             _STREAMS.append(torch.cuda.Stream())
+        work = queue.Queue()
+        for b in sorted(batches, key=lambda x: -x.cost()):
+            work.put(b)
         main = torch.cuda.current_stream()
         errors = []
 
@@ -250,11 +244,16 @@ def step(bins, i, emitter, timer=None, streams=1):
                 s = _STREAMS[k]
                 s.wait_stream(main)
                 with torch.cuda.stream(s):
-                    run_lane(ls[k])
+                    while not errors:
+                        try:
+                            b = work.get_nowait()
+                        except queue.Empty:
+                            break
+                        run_one(b)
             except BaseException as exc:            # re-raised on the calling thread
                 errors.append(exc)
 
-        th = [threading.Thread(target=worker, args=(k,)) for k in range(len(ls))]
+        th = [threading.Thread(target=worker, args=(k,)) for k in range(nl)]
         for t in th:
             t.start()
         for t in th:

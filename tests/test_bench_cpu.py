@@ -19,20 +19,3 @@ def test_pmc_traffic_for_the_default_workload():
 def test_pmc_traffic_only_for_its_workload():
     assert bench.pmc_traffic(bench.WORKLOAD + "; groups heavy only") is None
 
-
-class _B:
-    def __init__(self, g, c):
-        self.g, self.c = g, c
-
-    def cost(self):
-        return self.c
-
-
-def test_lanes_balance():
-    g = [object() for _ in range(5)]
-    bs = [_B(g[0], 9), _B(g[0], 8), _B(g[1], 5), _B(g[1], 4), _B(g[2], 7), _B(g[3], 3), _B(g[4], 2), _B(g[4], 1)]
-    ls = bench.lanes(bs, 2)
-    assert sorted(id(b) for l in ls for b in l) == sorted(id(b) for b in bs)
-    loads = sorted(sum(b.c for b in l) for l in ls)
-    assert loads == [19, 20]
-    assert bench.lanes(bs[:1], 3) == [bs[:1]]
