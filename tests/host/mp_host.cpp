@@ -90,6 +90,21 @@ static int64_t cauchy_t(const int32_t* rm, const int64_t* re, int64_t rps, int64
   return e;
 }
 
+// cauchy_entry2 (two rows of a group per lane): both rows' mantissas go to planes om (L x 2, entries 0 and 1),
+// exponents to oe[0..1] (row 2 only when has2)
+template <int L>
+static int64_t cauchy2_t(const int32_t* rm, const int64_t* re, int64_t rps, int64_t N, int64_t n, const int64_t* colg,
+                         const int32_t* coldg, int nc, int D1, int D2, int has2, const int32_t* hm, const int64_t* he,
+                         int64_t hps, int64_t hrow, int32_t* om, int64_t* oe) {
+  u32 R[L];
+  int64_t e1 = 0, e2 = 0;
+  mp::cauchy_entry2<L>(rm, re, rps, N, n, colg, coldg, nc, D1, D2, has2 != 0, hm, he, hps, hrow, R, e1, e2, om, 2, 0,
+                       1);
+  oe[0] = e1;
+  if (has2) oe[1] = e2;
+  return 0;
+}
+
 template <int L>
 static int64_t dot_t(const int32_t* am, int64_t aps, int64_t abase, int64_t ea, const int32_t* xm, const int64_t* xe,
                      int64_t xps, int64_t xbase, int64_t xstride, int S, uint32_t* r) {
@@ -178,6 +193,11 @@ int64_t hc_cauchy(int L, const int32_t* rm, const int64_t* re, int64_t rps, int6
                   const int32_t* coldg, int nc, int D, const int32_t* hm, const int64_t* he, int64_t hps, int64_t hrow,
                   int uni, uint32_t* r) {
   DISP(L, cauchy_t<LL>(rm, re, rps, N, n, colg, coldg, nc, D, hm, he, hps, hrow, uni, r))
+}
+int64_t hc_cauchy2(int L, const int32_t* rm, const int64_t* re, int64_t rps, int64_t N, int64_t n,
+                   const int64_t* colg, const int32_t* coldg, int nc, int D1, int D2, int has2, const int32_t* hm,
+                   const int64_t* he, int64_t hps, int64_t hrow, int32_t* om, int64_t* oe) {
+  DISP(L, cauchy2_t<LL>(rm, re, rps, N, n, colg, coldg, nc, D1, D2, has2, hm, he, hps, hrow, om, oe))
 }
 int64_t hc_dot(int L, const int32_t* am, int64_t aps, int64_t abase, int64_t ea, const int32_t* xm, const int64_t* xe,
                int64_t xps, int64_t xbase, int64_t xstride, int S, uint32_t* r) {

@@ -30,7 +30,7 @@ def lib():
     os.replace(tmp, SO)
     lb = ctypes.CDLL(SO)
     for f in (lb.hc_add, lb.hc_mul, lb.hc_norm, lb.hc_div, lb.hc_shift, lb.hc_cauchy, lb.hc_dot, lb.hc_mom,
-              lb.hc_tri, lb.hc_dot_staged):
+              lb.hc_tri, lb.hc_dot_staged, lb.hc_cauchy2):
         f.restype = I64
     return lb
 
@@ -116,6 +116,20 @@ def test_div_small_exact(lib, L):
         assert R.from_limbs(list(q)) == (x << 32) // d, (d, i, x)
 
 
+def test_div_every_divisor(lib):
+    """every divisor 0 < d < 2^15 with dividends whose digits meet the extremes: exact multiples of d (each
+    remainder 0, where a digit estimate below the exact quotient gives q - 1), remainders d - 1 at every step
+    (the largest fractional part), all-ones and the most negative value"""
+    L = 4
+    nb = 32 * L
+    for d in range(1, 1 << 15):
+        for x in (d * ((1 << (nb - 16)) - 1), ((1 << (nb - 1)) - 1), -(1 << (nb - 1)), -d * 12345,
+                  (((1 << (nb - 16)) // d) * d) - 1):
+            q = np.zeros(L + 1, np.uint32)
+            lib.hc_div(L, ptr(limbs(x, L)), d, ptr(q))
+            assert R.from_limbs(list(q)) == (x << 32) // d, (d, x)
+
+
 @pytest.mark.parametrize("L", [4, 34])
 def test_shifts(lib, L):
     rng = random.Random(300 + L)
@@ -163,6 +177,42 @@ def test_cauchy_entry(lib, L):
             e = lib.hc_cauchy(L, ptr(rm), ptr(re_), I64(nslot * N), I64(N), I64(n), ptr(colg), ptr(coldg), nc, D,
                               ptr(hm), ptr(he), I64(2 * N), I64(1), uni, ptr(r))
             assert got(r, e) == want, (trial, uni)
+
+
+@pytest.mark.parametrize("L", [4, 10, 34, 50])
+def test_cauchy_entry2(lib, L):
+    """two rows of a group per lane (the kernel of large levels) against the oracle for each row: a second row
+    or none, either row at a divisor 0 (the term left out for that row only), h_inf zero, every column zero"""
+    rng = random.Random(450 + L)
+    for trial in range(200):
+        nc = rng.randint(1, 12)
+        N, nslot = 3, nc + 2
+        vals = [rp(rng, L, spread=rng.choice([3, 200])) for _ in range(nslot * N)]
+        if trial % 13 == 0:
+            vals = [(0, R.EZ)] * len(vals)
+        rm = np.stack([limbs(v[0], L) for v in vals], 1).view(np.int32).copy()
+        re_ = np.array([v[1] for v in vals], np.int64)
+        colg = np.array(rng.sample(range(nslot), nc), np.int64)
+        big = trial % 2 == 1
+        coldg = np.array([rng.randint(-16000, 16000) if big else rng.randint(-300, 300) for _ in range(nc)], np.int32)
+        pick = lambda: int(coldg[rng.randrange(nc)])
+        D1 = pick() if trial % 4 == 1 else (rng.randint(-16000, 16000) if big else rng.randint(-300, 300))
+        D2 = pick() if trial % 4 == 2 else (rng.randint(-16000, 16000) if big else rng.randint(-300, 300))
+        has2 = trial % 7 != 3
+        hv = [rp(rng, L, spread=rng.choice([3, 300])) for _ in range(2 * N)]
+        n = rng.randrange(N)
+        if trial % 5 == 0:
+            hv[N + n] = (0, R.EZ)
+        hm = np.stack([limbs(v[0], L) for v in hv], 1).view(np.int32).copy()
+        he = np.array([v[1] for v in hv], np.int64)
+        xs = [vals[int(c) * N + n] for c in colg]
+        om = np.zeros((L, 2), np.int32)
+        oe = np.zeros(2, np.int64)
+        lib.hc_cauchy2(L, ptr(rm), ptr(re_), I64(nslot * N), I64(N), I64(n), ptr(colg), ptr(coldg), nc, D1, D2,
+                       int(has2), ptr(hm), ptr(he), I64(2 * N), I64(1), ptr(om), ptr(oe))
+        for row, D in enumerate((D1, D2) if has2 else (D1,)):
+            want = R.cauchy(xs, [D - int(c) for c in coldg], L, hv[N + n])
+            assert got(om[:, row].view(np.uint32), int(oe[row])) == want, (trial, row)
 
 
 @pytest.mark.parametrize("L", [2, 4, 10, 34, 50])
