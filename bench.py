@@ -479,11 +479,15 @@ def serving_stream(seconds, outdir, seed=7):
         for s in (1, -1):
             make(e, s, 1)
     torch.cuda.synchronize()
+    srv.batch_sizes.clear()
+    srv.mixed_batches = 0
     burst = [srv.submit(make(rng.choice(pool), rng.choice((1, -1)), k)) for k in range(24)]
     t0 = time.time()
     for r in burst:
         r.wait(600)
     cap = len(burst) / (time.time() - t0)
+    burst_batches, burst_mixed = list(srv.batch_sizes), srv.mixed_batches
+    srv.mixed_batches = 0
     srv.completed.clear()
     srv.batch_sizes.clear()
     log(f"serving capacity {cap:.2f} requests/s; streaming for {seconds:.0f} s")
@@ -506,7 +510,10 @@ def serving_stream(seconds, outdir, seed=7):
     q = lambda f: lat[min(len(lat) - 1, int(f * len(lat)))]
     return {"requests": len(reqs), "rate_per_s": rate, "capacity_per_s": cap, "p50_ms": q(0.5) * 1e3,
             "p99_ms": q(0.99) * 1e3, "blocks_per_s": sum(len(range(r.key[0] % 2, r.key[2] + 1, 2)) for r in reqs) / wall,
-            "mean_batch": float(np.mean(srv.batch_sizes)) if srv.batch_sizes else 0.0, "warmup_s": warm_s,
+            "mean_batch": float(np.mean(srv.batch_sizes)) if srv.batch_sizes else 0.0,
+            "burst_mean_batch": float(np.mean(burst_batches)) if burst_batches else 0.0,
+            "burst_mixed_operator_batches": burst_mixed, "mixed_operator_batches": srv.mixed_batches,
+            "warmup_s": warm_s,
             "operators": len(pool), "graph_replays": srv.replays,
             "graphs": sum(1 for g in srv.graphs.values() if g), "capture_failures": srv.capture_failures,
             "latency_to": "last spin file closed", "files_written": emitter.files}

@@ -143,13 +143,16 @@ __global__ __launch_bounds__(256) void k_cauchy(const int32_t* __restrict__ rm, 
 // product) X entries one output row combines; every output of the column shares them, so the workgroup
 // aligns each column once (exponent from its nonzero terms), converts each term once into 28-bit limbs in
 // LDS (thread (column, s) stages term s), and every thread then forms its output's dot product from LDS
-// (mp::small_dot_staged, the same bits as mp::small_dot).  Threads: s fastest within a column.
+// (mp::small_dot_staged, the same bits as mp::small_dot).  Threads: s fastest within a column.  kofs (may be
+// null): per right-hand side k an offset added to the task's constant row, so that right-hand sides of
+// different operators (their tables stacked into one) share the launch.
 template <int L>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L <= 34 ? 2 : 1))) void k_pole_r(const int32_t* __restrict__ Hm, const i64* __restrict__ He, i64 hps,
                                                 const int64_t* __restrict__ thrw, const int32_t* __restrict__ tR,
                                                 i64 T, int SL, int SR, int S, i64 B, const int32_t* __restrict__ R28,
                                                 const uint32_t* __restrict__ Rsg, const i64* __restrict__ Re,
-                                                i64 rps, int32_t* __restrict__ Ym, i64* __restrict__ Ye, i64 yps) {
+                                                i64 rps, const int32_t* __restrict__ kofs, int32_t* __restrict__ Ym,
+                                                i64* __restrict__ Ye, i64 yps) {
   constexpr int NL = mp::R28<L>::NL;
   extern __shared__ int32_t st[];
   const int CPB = 256 / SR;                   // columns (task, a, k) per block, SR outputs b0 each
@@ -176,7 +179,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L <= 34 ? 2
     }
     __syncthreads();
     if (act) {
-      const i64 r = tR[task];
+      const i64 r = tR[task] + (kofs ? kofs[k] : 0);
       u32 out[L];
       const i64 e = mp::small_dot_staged<L>(R28, Rsg, rps, (r * S + b0) * S, Re[r * S + b0],
                                             st + lc * SR * (NL + 1), SR, E, out);
@@ -193,7 +196,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L <= 34 ? 2
                                                 const int32_t* __restrict__ tQL, const int64_t* __restrict__ tslot,
                                                 i64 T, int SL, int SR, int S, i64 B, const int32_t* __restrict__ Q28,
                                                 const uint32_t* __restrict__ Qsg, const i64* __restrict__ Qe, i64 qps,
-                                                int32_t* __restrict__ rm, i64* __restrict__ re, i64 rps) {
+                                                const int32_t* __restrict__ kofs, int32_t* __restrict__ rm,
+                                                i64* __restrict__ re, i64 rps) {
   constexpr int NL = mp::R28<L>::NL;
   extern __shared__ int32_t st[];
   const int CPB = 256 / SL;                   // columns (task, b0, k) per block, SL outputs a0 each
@@ -220,7 +224,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(L <= 34 ? 2
     }
     __syncthreads();
     if (act) {
-      const i64 q = tQL[task];
+      const i64 q = tQL[task] + (kofs ? kofs[k] : 0);
       u32 out[L];
       const i64 e = mp::small_dot_staged<L>(Q28, Qsg, qps, (q * S + a0) * S, Qe[q * S + a0],
                                             st + lc * SL * (NL + 1), SL, E, out);
@@ -271,16 +275,21 @@ int b3d_pole_level(int L, int SL, int SR, int S, int64_t B, int64_t T, const int
                    int64_t hps, const int64_t* thrw, const int32_t* tR, const int32_t* tQL, const int64_t* tslot,
                    const int32_t* R28, const uint32_t* Rsg, const int64_t* Re, int64_t rtps, const int32_t* Q28,
                    const uint32_t* Qsg, const int64_t* Qe, int64_t qps, const int8_t* Rd8, const int32_t* Rcorr,
-                   const int8_t* Qd8, const int32_t* Qcorr, int32_t* Ym, int64_t* Ye, int64_t yps, int32_t* rm,
-                   int64_t* re, int64_t rps, void* stream) {
+                   const int8_t* Qd8, const int32_t* Qcorr, const int32_t* kR, const int32_t* kQ, int32_t* Ym,
+                   int64_t* Ye, int64_t yps, int32_t* rm, int64_t* re, int64_t rps, void* stream) {
   if (T <= 0 || B <= 0) return 0;
+  if ((kR == nullptr) != (kQ == nullptr)) {
+    b3d::set_error("b3d_pole_level: give both per-column offsets kR and kQ or neither");
+    return 2;
+  }
   if (S < 1 || S > 16 || SL < 1 || SR < 1 || SL > S || SR > S) {
     b3d::set_error("b3d_pole_level: need 1 <= SL, SR <= S <= 16");
     return 2;
   }
   if (b3d::check_planes("b3d_pole_level", {hps, rtps, qps, yps, rps})) return 2;
   hipStream_t s = (hipStream_t)stream;
-  if (Rd8 && Rcorr && Qd8 && Qcorr && SL == S && SR == S && (int64_t)S * B >= 32 && S <= b3d::pole_mma_max_s(L)) {
+  if (Rd8 && Rcorr && Qd8 && Qcorr && !kR && SL == S && SR == S && (int64_t)S * B >= 32 &&
+      S <= b3d::pole_mma_max_s(L)) {
     // square structure blocks, at least 32 rows sharing each constant row: int8 MFMA products
     if (b3d::pole_mma(L, Hm, He, hps, thrw, tR, Rd8, Rcorr, Re, nullptr, T, S, B, Ym, Ye, yps, s)) return 1;
     return b3d::pole_mma(L, Ym, Ye, yps, nullptr, tQL, Qd8, Qcorr, Qe, tslot, T, S, B, rm, re, rps, s);
@@ -289,9 +298,9 @@ int b3d_pole_level(int L, int SL, int SR, int S, int64_t B, int64_t T, const int
   const int nl = b3d_limbs28(L);
   const size_t lds = 256 * (size_t)(nl + 1) * sizeof(int32_t);
   const int64_t colR = T * SL * B, colL = T * SR * B;
-  B3D_DISPATCH_L(L, k_pole_r<LL><<<dim3(grid_for(colR * SR, 256 / SR * SR)), dim3(256), lds, s>>>(Hm, He, hps, thrw, tR, T, SL, SR, S, B, R28, Rsg, Re, rtps, Ym, Ye, yps));
+  B3D_DISPATCH_L(L, k_pole_r<LL><<<dim3(grid_for(colR * SR, 256 / SR * SR)), dim3(256), lds, s>>>(Hm, He, hps, thrw, tR, T, SL, SR, S, B, R28, Rsg, Re, rtps, kR, Ym, Ye, yps));
   if (b3d::check_launch("b3d_pole_level (R)")) return 1;
-  B3D_DISPATCH_L(L, k_pole_l<LL><<<dim3(grid_for(colL * SL, 256 / SL * SL)), dim3(256), lds, s>>>(Ym, Ye, yps, tQL, tslot, T, SL, SR, S, B, Q28, Qsg, Qe, qps, rm, re, rps));
+  B3D_DISPATCH_L(L, k_pole_l<LL><<<dim3(grid_for(colL * SL, 256 / SL * SL)), dim3(256), lds, s>>>(Ym, Ye, yps, tQL, tslot, T, SL, SR, S, B, Q28, Qsg, Qe, qps, kQ, rm, re, rps));
   return b3d::check_launch("b3d_pole_level (L)");
 }
 

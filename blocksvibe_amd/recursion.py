@@ -202,11 +202,13 @@ class Recursion:
                          len(wrow), rpl))
         return tabs
 
-    def run(self, h_inf, hook=None, prof=None, res=None, timer=None):
+    def run(self, h_inf, hook=None, prof=None, res=None, timer=None, operators=None):
         """h_inf: MPF (2, order + 1, Jh, SL SR, B) (structure entry (a, b) at a SR + b).  Returns the residue
         slot buffer MPF (nphys + 1, SL SR, B) (physical slots; the last one is the zero slot).  hook(m, res)
         runs after each level (e.g. the derivative moments).  timer: a list that receives (start event, end
-        event, algorithmic bytes) of every Cauchy launch, events recorded on the launch stream."""
+        event, algorithmic bytes) of every Cauchy launch, events recorded on the launch stream.  operators: an
+        Operators of this plan and L whose right-hand sides are the columns of h_inf (each column with its own
+        operator's M matrices); None: every column with this Recursion's."""
         plan, L, Sm = self.plan, self.L, self.plan.Smax
         SL, SR, S2 = plan.SLm, plan.SRm, plan.S2
         assert h_inf.shape[:4] == (2, plan.order + 1, plan.Jh, S2), (h_inf.shape, plan.order, plan.Jh, S2)
@@ -230,6 +232,12 @@ class Recursion:
         st = mpf.stream_ptr()
         P = lambda t: t.data_ptr()
         waves = self.wave_tables(N) if self.group_waves else [(None, None, None, 0, 1)] * len(self.levels)
+        if operators is not None:
+            if operators.kR.numel() != B or operators.plan is not plan or operators.L != L:
+                raise _lib.B3DError("operators: one operator per right-hand side, of this plan and limb count")
+            C, kR, kQ, mma = operators, P(operators.kR), P(operators.kQ), False
+        else:
+            C, kR, kQ, mma = self, 0, 0, self.mma
         for lv, wt in zip(self.levels, waves):
             if lv is None:
                 continue
@@ -250,10 +258,10 @@ class Recursion:
                 t1 = time.time()
                 prof["cauchy"] = prof.get("cauchy", 0.0) + t1 - t0
             _lib.call("b3d_pole_level", L, SL, SR, Sm, B, lv["T"], P(H.m), P(H.e), H.numel(), P(lv["t_hrow"]),
-                      P(lv["t_R"]), P(lv["t_qL"]), P(lv["t_slot"]), P(self.R28), P(self.Rsg), P(self.Ralign_e),
-                      self.R28[0].numel(), P(self.QL28), P(self.QLsg), P(self.QLalign_e), self.QL28[0].numel(),
-                      P(self.Rd8) if self.mma else 0, P(self.Rcorr) if self.mma else 0,
-                      P(self.QLd8) if self.mma else 0, P(self.QLcorr) if self.mma else 0,
+                      P(lv["t_R"]), P(lv["t_qL"]), P(lv["t_slot"]), P(C.R28), P(C.Rsg), P(C.Ralign_e),
+                      C.R28[0].numel(), P(C.QL28), P(C.QLsg), P(C.QLalign_e), C.QL28[0].numel(),
+                      P(self.Rd8) if mma else 0, P(self.Rcorr) if mma else 0,
+                      P(self.QLd8) if mma else 0, P(self.QLcorr) if mma else 0, kR, kQ,
                       P(Y.m), P(Y.e), Y.numel(), P(res.m), P(res.e), res.numel(), st)
             if prof is not None:
                 torch.cuda.synchronize()
@@ -269,3 +277,31 @@ class Recursion:
     def slot(self, p, m, jidx, pidx):
         """physical slot of residue (p, m, jidx, pidx) (meaningful after run() only without slot reuse)"""
         return self.plan.phys_slot(p * self.plan.nslots + self.plan.offset(m, jidx) + pidx)
+
+
+class Operators:
+    """Right-hand sides of several operators of one plan and limb count in one recursion: the pole step's
+    constant tables (R rows, q L rows: the only part of the recursion that depends on the M matrices) of every
+    operator stacked into one, operator by operator, and per right-hand side the row offsets of its operator
+    (b3d_pole_level's kR, kQ).  recs: Recursion objects of one plan and L (Recursion.with_matrices);
+    col_op: per right-hand side (column of h_inf) the index of its operator in recs.  The Cauchy step, the
+    moments and everything after the recursion do not depend on M, so the columns of a batch may mix
+    operators freely and each keeps the bits it has alone."""
+
+    def __init__(self, recs, col_op):
+        r0 = recs[0]
+        if any(r.plan is not r0.plan or r.L != r0.L for r in recs):
+            raise _lib.B3DError("Operators: every operator must share one plan and limb count")
+        if not col_op or min(col_op) < 0 or max(col_op) >= len(recs):
+            raise _lib.B3DError("Operators: col_op indexes recs")
+        self.plan, self.L = r0.plan, r0.L
+        self.R28 = torch.cat([r.R28 for r in recs], 1)
+        self.Rsg = torch.cat([r.Rsg for r in recs], 0)
+        self.Ralign_e = torch.cat([r.Ralign_e for r in recs], 0)
+        self.QL28 = torch.cat([r.QL28 for r in recs], 1)
+        self.QLsg = torch.cat([r.QLsg for r in recs], 0)
+        self.QLalign_e = torch.cat([r.QLalign_e for r in recs], 0)
+        nR, nQ = r0.R28.shape[1], r0.QL28.shape[1]
+        dev = r0.R28.device
+        self.kR = torch.tensor([i * nR for i in col_op], dtype=torch.int32, device=dev)
+        self.kQ = torch.tensor([i * nQ for i in col_op], dtype=torch.int32, device=dev)

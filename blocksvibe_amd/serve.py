@@ -2,9 +2,10 @@
 
 Requests (one blocks_3d task each: operator, four-point sign, lambda orders, h_inf) arrive at any time.
 Worker threads, one per HIP stream, repeatedly take the oldest waiting request and every waiting request
-compatible with it (same operator, sign and lambda orders), up to what the HBM budget still admits, and
-run them as one batch: one recursion per lambda order over all their structures, the moments, the
-numerator GEMMs and the write_output GEMM.  Requests that arrive while batches are on the GPU join the
+compatible with it (same operator shape -- plan, limb count, Lambda, kept pole order -- sign and lambda
+orders, any M matrices), up to what the HBM budget still admits, and run them as one batch: one recursion per
+lambda order over all their structures (the pole step reads each right-hand side's own operator tables,
+recursion.Operators), the moments, the numerator GEMMs and the write_output GEMM.  Requests that arrive while batches are on the GPU join the
 next batch as soon as a worker and memory are free, and batches on different streams overlap on the
 device.  Plans, uploaded level tables, operator constants (M, qL, R tables) and write_output maps are cached
 on the GPU across requests.  A batch shape (operator, sign, batch size, coordinates) that has run once is
@@ -34,7 +35,7 @@ from . import _lib, mpf
 from .derivatives import Derivs
 from .output import cached as output_cached
 from .plan import Plan
-from .recursion import Recursion
+from .recursion import Operators, Recursion
 
 
 class Request:
@@ -211,6 +212,8 @@ class Server:
         self.lock = threading.Condition()
         self.queue = collections.deque()
         self.ops, self.plans, self.outs = {}, {}, {}
+        self.stacked = {}           # recursion.Operators of mixed-operator batches, by the columns' operators
+        self.mixed_batches = 0      # batches whose requests belong to more than one operator
         self.setup_lock = threading.Lock()
         self.stop = False
         self.completed = []
@@ -292,7 +295,10 @@ class Server:
                     for r in list(self.queue)[1:]:
                         if len(batch) >= nmax:
                             break
-                        if r.key == head.key and r.sign == head.sign and len(r.hs) == len(head.hs):
+                        # the same shape (every key field but the operator id), sign and lambda orders; the
+                        # operator of another M must already be built (otherwise it heads a batch later)
+                        if (r.key[:-1] == head.key[:-1] and r.sign == head.sign and len(r.hs) == len(head.hs)
+                                and (r.key == head.key or r.key in self.ops)):
                             batch.append(r)
                     wait = self.window - (time.time() - head.t_arrival)
                     if wait > 0 and len(batch) < nmax and not self.stop:
@@ -343,9 +349,27 @@ class Server:
             self.completed.append(r)
         r.done.set()
 
-    def _pipeline(self, op, key, sign, coords, hs):
+    def _operators(self, batch):
+        """recursion.Operators of a batch whose requests belong to more than one operator (None otherwise),
+        cached per assignment of operators to columns (a captured batch keeps reading the same tables)"""
+        keys = [r.key for r in batch]
+        if all(k == keys[0] for k in keys):
+            return None
+        with self.setup_lock:
+            ops = self.stacked.get(tuple(keys))
+            if ops is None:
+                order = list(dict.fromkeys(keys))
+                ops = Operators([self.ops[k].rec for k in order],
+                                [order.index(k) for r in batch for k in [r.key] * r.hs[0].shape[-1]])
+                self.stacked[tuple(keys)] = ops
+                while len(self.stacked) > 64:
+                    self.stacked.pop(next(iter(self.stacked)))
+            return ops
+
+    def _pipeline(self, op, key, sign, coords, hs, operators=None):
         """device work of one batch: per lambda order a recursion over all right-hand sides with the moment
-        hook, the numerators, then the write_output map per (coordinate, Delta_1+2)"""
+        hook (operators: the columns' own operators when they differ), the numerators, then the write_output
+        map per (coordinate, Delta_1+2)"""
         tj12, tj43, tjmax, order, Lam, kept, L, pp, op_id = key
         p0 = (1 - sign) // 2
         der = op.derivs_for(p0)
@@ -354,7 +378,7 @@ class Server:
         for i, h in enumerate(hs):
             dn = der.for_order(Lam - (p0 + 2 * i))
             dn.start(B)
-            op.rec.run(h, hook=dn.accumulate)
+            op.rec.run(h, hook=dn.accumulate, operators=operators)
             nums.append(dn.finish(h))
         return {(c, d): self._output(c, Lam, p0, d, L).apply_raw(nums) for c, d in coords}
 
@@ -362,13 +386,16 @@ class Server:
         t0 = time.time()
         head = batch[0]
         op = self._operator(head.key, head.M)
+        operators = self._operators(batch)
+        if operators is not None:
+            self.mixed_batches += 1
         for r in batch:
             r.t_start = t0
         coords = tuple(sorted({(c, r.d12) for r in batch for c in r.coords}))
         nh = len(head.hs)
         inputs = lambda: [mpf.cat([r.hs[i] for r in batch], 4) if len(batch) > 1 else head.hs[i] for i in range(nh)]
-        gkey = (head.key, head.sign, len(batch), nh, coords)
-        fn = lambda x: self._pipeline(op, head.key, head.sign, coords, x)
+        gkey = (tuple(r.key for r in batch), head.sign, nh, coords)
+        fn = lambda x: self._pipeline(op, head.key, head.sign, coords, x, operators)
         graph, capture = None, False
         if self.use_graphs:
             with self.graph_lock:

@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-#define B3D_ABI_VERSION 14
+#define B3D_ABI_VERSION 15
 
 /* Last error message of the calling thread (empty if none). */
 const char* b3d_last_error(void);
@@ -100,14 +100,17 @@ int b3d_cauchy_level(int L, const int32_t* rm, const int64_t* re, int64_t rps, i
  * masks Rsg/Qsg (ntab, S, S), with row exponents Re/Qe (ntab, S); and in the form b3d_pole_tables makes
  * (digits Rd8/Qd8, corrections Rcorr/Qcorr; may be null).  With the digit tables, SL = SR = S, S*B >= 32 and S
  * small enough for the MFMA path's LDS staging (6 at L = 34) the products run on int8 MFMA, otherwise on the
- * VALU; both give the same bits.  Y: scratch (L, T, SR, SL, B), plane stride yps.  The reference pads every
- * residue to Smax x Smax (recursion_gpu.py:483-510); here only the SL x SR entries are stored and computed. */
+ * VALU; both give the same bits.  kR, kQ (both or neither; may be null, and then the MFMA path is not taken):
+ * per right-hand side k an offset added to every task's row index into Ralign resp. QLalign, so that the right-hand
+ * sides of several operators of one plan, with their tables stacked into one, run in one launch.  Y: scratch
+ * (L, T, SR, SL, B), plane stride yps.  The reference pads every residue to Smax x Smax and runs one operator per
+ * launch (recursion_gpu.py:483-510); here only the SL x SR entries are stored and computed. */
 int b3d_pole_level(int L, int SL, int SR, int S, int64_t B, int64_t T, const int32_t* Hm, const int64_t* He,
                    int64_t hps, const int64_t* thrw, const int32_t* tR, const int32_t* tQL, const int64_t* tslot,
                    const int32_t* R28, const uint32_t* Rsg, const int64_t* Re, int64_t rtps, const int32_t* Q28,
                    const uint32_t* Qsg, const int64_t* Qe, int64_t qps, const int8_t* Rd8, const int32_t* Rcorr,
-                   const int8_t* Qd8, const int32_t* Qcorr, int32_t* Ym, int64_t* Ye, int64_t yps, int32_t* rm,
-                   int64_t* re, int64_t rps, void* stream);
+                   const int8_t* Qd8, const int32_t* Qcorr, const int32_t* kR, const int32_t* kQ, int32_t* Ym,
+                   int64_t* Ye, int64_t yps, int32_t* rm, int64_t* re, int64_t rps, void* stream);
 
 /* Bytes per constant entry in the digit tables of b3d_pole_tables (4L + 4), or -1 for an unsupported L. */
 int b3d_pole_digits_bytes(int L);

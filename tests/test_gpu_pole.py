@@ -51,7 +51,7 @@ def test_pole_mma_matches_valu(L, S, B):
         res = mpf.zeros((T + 2, S * S, B), L)
         tb = [P(Rd8), P(Rc), P(Qd8), P(Qc)] if tables else [0, 0, 0, 0]
         _lib.call("b3d_pole_level", L, S, S, S, B, T, P(H.m), P(H.e), H.numel(), P(thrw), P(tR), P(tQL), P(tslot),
-                  P(R28), P(Rsg), P(Re), R28[0].numel(), P(Q28), P(Qsg), P(Qe), Q28[0].numel(), *tb,
+                  P(R28), P(Rsg), P(Re), R28[0].numel(), P(Q28), P(Qsg), P(Qe), Q28[0].numel(), *tb, 0, 0,
                   P(Y.m), P(Y.e), Y.numel(), P(res.m), P(res.e), res.numel(), mpf.stream_ptr())
         torch.cuda.synchronize()
         outs.append((Y.m.cpu(), Y.e.cpu(), res.m.cpu(), res.e.cpu()))
@@ -90,7 +90,7 @@ def test_pole_rectangular_matches_padded(L, SL, SR, S, B):
         res = mpf.zeros((T + 2, sl, sr, B), L)
         _lib.call("b3d_pole_level", L, sl, sr, S, B, T, P(H.m), P(H.e), H.numel(), P(thrw), P(tR), P(tQL),
                   P(tslot), P(R28), P(Rsg), P(Re), R28[0].numel(), P(Q28), P(Qsg), P(Qe), Q28[0].numel(), 0, 0, 0, 0,
-                  P(Y.m), P(Y.e), Y.numel(), P(res.m), P(res.e), res.numel(), mpf.stream_ptr())
+                  0, 0, P(Y.m), P(Y.e), Y.numel(), P(res.m), P(res.e), res.numel(), mpf.stream_ptr())
         torch.cuda.synchronize()
         outs.append(res)
     got, pad = outs

@@ -191,3 +191,27 @@ def test_recursion_golden_vectors(i, L):
     bits = (worst.denominator.bit_length() - worst.numerator.bit_length()) if worst else float("inf")
     print(f"max error relative to the (pole, spin) block: 2^-{bits} (bound 2^-{32 * L - 64})")
     assert bits > 32 * L - 64
+
+
+@pytest.mark.parametrize("L,tj12,tj43", [(10, 4, 2), (34, 8, 8)])
+def test_operators_of_one_plan_in_one_recursion(L, tj12, tj43):
+    """right-hand sides of two operators of one plan (other M matrices) in one recursion (recursion.Operators,
+    the pole step's per-column table offsets): every column bit-identical to its own operator's recursion run
+    alone, and the operators' results differ (so the columns really read their own tables)"""
+    from blocksvibe_amd import mpf
+    from blocksvibe_amd.plan import Plan
+    from blocksvibe_amd.recursion import Operators, Recursion
+    plan = Plan(tj12, tj43, 4, 6, ((0, 1), (1, 0)))
+    ra = Recursion(plan, None, L, seed=21)
+    rb = ra.with_matrices(None, seed=22)
+    h = mpf.random_device((2, plan.order + 1, plan.Jh, plan.S2, 4), L, seed=23)
+    cols = [0, 1, 1, 0]
+    mixed = ra.run(h, operators=Operators([ra, rb], cols))
+    mixed = (mixed.m.cpu(), mixed.e.cpu())
+    alone = []
+    for r in (ra, rb):
+        x = r.run(h)
+        alone.append((x.m.cpu(), x.e.cpu()))
+    for k, i in enumerate(cols):
+        assert torch.equal(mixed[0][..., k], alone[i][0][..., k]) and torch.equal(mixed[1][..., k], alone[i][1][..., k]), k
+    assert not torch.equal(alone[0][0], alone[1][0])

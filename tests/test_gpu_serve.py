@@ -1,7 +1,8 @@
 """The serving loop (blocksvibe_amd.serve): requests of several operators and both signs, first one at a
 time, then all submitted at once so that they are batched and overlap on two streams.  Every result must be
 bit-identical alone and in the mix (each right-hand-side column is computed independently), batches of
-more than one request must have formed, and latencies are recorded for every request."""
+more than one request must have formed, requests of two operators of one shape (other M matrices) must have
+shared a batch, and latencies are recorded for every request."""
 import pytest
 import torch
 
@@ -44,6 +45,7 @@ def test_alone_and_in_mix_identical():
     mix = [_host(r.wait(300)) for r in rs]
     srv.close()
     assert max(srv.batch_sizes) > 1, srv.batch_sizes
+    assert srv.stacked, "operators 0 and 2 (one shape, other M matrices) never shared a batch"
     for a, b in zip(alone, mix):
         assert a.keys() == b.keys()
         for k in a:
