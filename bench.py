@@ -311,20 +311,51 @@ def _random_pairs(rng, n, L, spread):
     return out
 
 
-def cpu_baseline(groups, seconds=12.0, sample_s=10.0):
+def _host_threads():
+    return max(1, len(os.sched_getaffinity(0)))
+
+
+def chain_products(batch):
+    """multiprecision multiply-adds of the stages after the recursion for one batch (all its lambda orders), in
+    units of one L x L limb product: the moments (each staged residue times r_c^n, then R + 1 products by the
+    exact n^k of KL limbs), the derivative GEMMs of Derivs.finish (c = Wh H, the triangular a_i of TL-limb
+    integers, G = Vand v, Hm = Cb G, P, the numerators Anum [a | P]) and the write_output GEMM"""
+    pl, B, L = batch.g.plan, NQ_PER_SIGN, batch.g.L
+    S2 = pl.S2
+    Nc = 2 * S2 * B
+    tot, dmax, jo = 0.0, 0, 1
+    for n in batch.ns:
+        dn = batch.der.for_order(LAMBDA - n)
+        R1, Rw, Jo, NP, Dout = dn.R + 1, max(dn.R, 1), dn.Jo, dn.NP, dn.Dout
+        tl = dn.tri[0] / L if dn.tri is not None else 1.0
+        staged = sum(len(lv[0]) for lv in dn.levels if lv is not None) * S2 * B
+        tot += staged * (1.0 + R1 * dn.KL / L)                                     # moments
+        tot += R1 * (ORDER + 1) * Jo * Nc                                           # c
+        tot += Jo * NP * Nc * R1 * (R1 + 1) / 2 * tl                                # a (triangular)
+        tot += Jo * Rw * NP * R1 * Nc + R1 ** 3 * Jo * Nc + R1 ** 2 * Jo * Nc       # G, Hm, P
+        tot += Jo * Dout * (NP + R1) * R1 * Nc                                      # numerators
+        dmax, jo = max(dmax, Dout), Jo
+    W = batch.out.W
+    tot += W.shape[0] * W.shape[1] * jo * 2 * S2 * B * dmax                          # write_output
+    return tot
+
+
+def cpu_baseline(groups, seconds=12.0, sample_s=10.0, json_bytes=None, json_rate=None):
     """The oracle's C restatement of the recursion (oracle/c/mpf_c.c through oracle/c_chain.py: the level loop
     of Cauchy and pole steps, bit-identical to the GPU kernels' spec) on every host thread of this process.
 
     Measured: one whole recursion of a real operator group of the workload at its real shape (order 60,
     2 j_max 60, 1024 bits, the 11 structures of one sign as right-hand sides), the largest group whose
     predicted time fits sample_s.  The other groups are priced by a per-unit model (Cauchy entry, Cauchy term,
-    pole dot product, each timed on the same threads) calibrated by that run.  Derivative, output and text
-    stages are not counted, so the figure is an upper bound on the CPU rate."""
+    pole dot product, each timed on the same threads) calibrated by that run.  The stages after the recursion
+    (moments, derivative and write_output GEMMs: chain_products) are priced at the C dot product's time per L x
+    L limb product on the same threads, and the JSON text at the library's multithreaded host formatter's
+    measured rate (json_rate bytes/s over json_bytes per pass), when given.  File writes are not counted."""
     import numpy as np
     from oracle import c_chain as CC
     from oracle import c_ref as C
     L = 34
-    th = max(1, len(os.sched_getaffinity(0)))
+    th = _host_threads()
     part = seconds / 8
     t1 = C.time_unit(0, L, 1, th, part) * 1
     t4 = C.time_unit(0, L, 4, th, part) * 4
@@ -348,7 +379,12 @@ def cpu_baseline(groups, seconds=12.0, sample_s=10.0):
     CC.recursion(pl, Mtab, hm, he, L, NQ_PER_SIGN, threads=th, timing=tm)
     measured = tm["seconds"]
     ratio = measured / (per_rhs(pl) * NQ_PER_SIGN)
-    tot = ratio * sum(per_rhs(x.plan) * NQ_PER_SIGN * len(bt.ns) for x in groups for bt in x.batches)
+    rec_s = ratio * sum(per_rhs(x.plan) * NQ_PER_SIGN * len(bt.ns) for x in groups for bt in x.batches)
+    tmac = tdot[5] / 5
+    prods = sum(chain_products(bt) for x in groups for bt in x.batches)
+    post_s = prods * tmac
+    json_s = json_bytes / json_rate if json_bytes and json_rate else 0.0
+    tot = rec_s + post_s + json_s
     blocks = sum(bt.blocks() for x in groups for bt in x.batches)
     sample_blocks = g.batches[0].blocks() / (measured * len(g.batches[0].ns))
     model = ""
@@ -360,14 +396,28 @@ def cpu_baseline(groups, seconds=12.0, sample_s=10.0):
     except OSError:
         pass
     return {"value": blocks / tot, "unit": "blocks/s", "cores": th, "kind": "port", "cpu_model": model,
+            "host_cpus": os.cpu_count(),
+            "pass_s": tot, "recursion_s": rec_s, "post_recursion_s": post_s, "json_text_s": json_s,
+            "post_recursion_products": prods, "s_per_limb_product": tmac,
             "measured_group": f"{g.tj12},{g.tj43}", "measured_recursion_s": measured,
             "measured_group_blocks_per_s": sample_blocks, "model_calibration": ratio,
             "sample": f"oracle/c/mpf_c.c level loop (oracle/c_chain.py) on {th} threads: one whole recursion of "
                       f"operator group ({g.tj12},{g.tj43}) at order {ORDER}, 1024 bits, {NQ_PER_SIGN} right-hand "
                       f"sides timed ({measured:.1f} s); the other groups priced by per-unit costs (Cauchy entry "
                       f"{a * 1e9:.0f} ns + {b * 1e9:.0f} ns per term, 5-term pole dot {tdot[5] * 1e9:.0f} ns, "
-                      f"{seconds:.0f} s of timing) scaled by that run ({ratio:.2f}x); derivative, output and text "
-                      f"stages not counted (blocks_3d itself is not on this machine)"}
+                      f"{seconds:.0f} s of timing) scaled by that run ({ratio:.2f}x); moments, derivative and "
+                      f"write_output GEMMs priced at the C dot's {tmac * 1e9:.0f} ns per 34 x 34 limb product "
+                      f"({prods:.3g} per pass); JSON text at the library's host formatter's measured rate; file "
+                      f"writes not counted (blocks_3d itself is not on this machine)"}
+
+
+def leg(res, key, fn):
+    """res[key] = fn(), or the error it raised: the measurements beside the headline never cost the JSON line"""
+    try:
+        res[key] = fn()
+    except Exception as e:                    # noqa: BLE001 -- reported in the JSON, the run goes on
+        res[key] = {"error": f"{type(e).__name__}: {str(e).splitlines()[0] if str(e) else ''}"}
+    log(f"{key}: {res[key]}")
 
 
 def emission_check(batch, emitter_cls, outdir):
@@ -386,10 +436,14 @@ def emission_check(batch, emitter_cls, outdir):
     m = r.m.reshape(L, -1, D).index_select(1, sel).cpu().numpy().reshape(L, -1)
     e = r.e.reshape(-1, D).index_select(0, sel).cpu().numpy().reshape(-1)
     local = EM.FileJob(job.path, job.head, job.skel, np.arange(len(job.polys)))
+    EM.assemble_host(m, e, D, batch.p, batch.eps, [local])                # warm
+    th = time.time()
     want = EM.assemble_host(m, e, D, batch.p, batch.eps, [local])[0]
+    th = time.time() - th
     got = open(job.path, "rb").read()
     os.unlink(job.path)
-    return {"file_bytes": len(got), "identical_to_host_formatter": got == want}
+    return {"file_bytes": len(got), "identical_to_host_formatter": got == want, "host_formatter_s": th,
+            "host_formatter_gb_per_s": len(want) / 1e9 / max(th, 1e-9), "host_formatter_threads": _host_threads()}
 
 
 def serving_stream(seconds, outdir, seed=7):
@@ -429,7 +483,7 @@ def serving_stream(seconds, outdir, seed=7):
             pool.append({"key": (tj12, tj43, tjmax, order, Lam, 2 * nmax + 2, L, PP, 10 * i + variant), "plan": plan,
                          "ext": ext, "prec": prec, "deltas": d12s})
     srv = Server(streams=2)
-    emitter = Emitter(writers=16, buffers=4, unlink=True)
+    emitter = Emitter(writers=16, buffers=4, unlink=True, gate=srv.gate.device)
     hcache, counter = {}, [0]
 
     def make(entry, sign, k):
@@ -470,9 +524,29 @@ def serving_stream(seconds, outdir, seed=7):
         return Request(entry["key"], sign, hcache[hk], coords=("xt",), d12="0", sink=sink)
 
     t0 = time.time()
+    failed = []
+
+    def wait(r):
+        """a request's end; a failed one is counted (its error kept) and the stream goes on"""
+        try:
+            r.wait(600)
+        except Exception as e:                # noqa: BLE001 -- reported in the JSON
+            failed.append(f"{type(e).__name__}: {str(e).splitlines()[0] if str(e) else ''}")
+
+    try:
+        return _serve(srv, emitter, rng, pool, make, seconds, wait, failed)
+    finally:
+        srv.close()                               # a second close is a no-op
+        emitter.close()
+
+
+def _serve(srv, emitter, rng, pool, make, seconds, wait, failed):
+    import numpy as np
+    import torch
+    t0 = time.time()
     warm = [srv.submit(make(e, s, 0)) for e in pool for s in (1, -1)]
     for i, r in enumerate(warm):
-        r.wait(600)
+        wait(r)
         log(f"serving warm-up {i + 1}/{len(warm)} at {time.time() - t0:.1f} s")
     warm_s = time.time() - t0
     for e in pool:                                  # every input of the stream exists before it starts
@@ -484,7 +558,7 @@ def serving_stream(seconds, outdir, seed=7):
     burst = [srv.submit(make(rng.choice(pool), rng.choice((1, -1)), k)) for k in range(24)]
     t0 = time.time()
     for r in burst:
-        r.wait(600)
+        wait(r)
     cap = len(burst) / (time.time() - t0)
     burst_batches, burst_mixed = list(srv.batch_sizes), srv.mixed_batches
     srv.mixed_batches = 0
@@ -502,11 +576,11 @@ def serving_stream(seconds, outdir, seed=7):
         else:
             time.sleep(min(0.005, nxt - now))
     for r in reqs:
-        r.wait(600)
+        wait(r)
     wall = time.time() - t0
     srv.close()
-    emitter.close()
-    lat = sorted(r.t_done - r.t_arrival for r in reqs)
+    emitter.drain()
+    lat = sorted(r.t_done - r.t_arrival for r in reqs if r.error is None)
     q = lambda f: lat[min(len(lat) - 1, int(f * len(lat)))]
     return {"requests": len(reqs), "rate_per_s": rate, "capacity_per_s": cap, "p50_ms": q(0.5) * 1e3,
             "p99_ms": q(0.99) * 1e3, "blocks_per_s": sum(len(range(r.key[0] % 2, r.key[2] + 1, 2)) for r in reqs) / wall,
@@ -516,7 +590,8 @@ def serving_stream(seconds, outdir, seed=7):
             "warmup_s": warm_s,
             "operators": len(pool), "graph_replays": srv.replays,
             "graphs": sum(1 for g in srv.graphs.values() if g), "capture_failures": srv.capture_failures,
-            "latency_to": "last spin file closed", "files_written": emitter.files}
+            "latency_to": "last spin file closed", "files_written": emitter.files, "failed_requests": len(failed),
+            "first_failure": failed[0] if failed else None}
 
 
 def fs_type(path):
@@ -598,9 +673,9 @@ def main():
         res["roofline"] = roofline(timer)
         log(f"device only: {nb / td:.1f} blocks/s; roofline {res['roofline']}")
     heavy = max(groups, key=lambda g: g.plan.S2)
-    res["emission_check"] = emission_check(heavy.batches[0], Emitter, outdir)
-    log(f"emission check: {res['emission_check']}")
-    if args.plain_tasks > 0:
+    leg(res, "emission_check", lambda: emission_check(heavy.batches[0], Emitter, outdir))
+
+    def plain():
         torch.cuda.synchronize()
         tb = time.time()
         step([heavy.batches], 0, None)
@@ -611,18 +686,23 @@ def main():
         tp = time.time() - tp
         nbp = n * (TJMAX // 2 + 1)
         batched = sum(b.blocks() for b in heavy.batches) / tb
-        res["plain_path"] = {"group": f"{heavy.tj12},{heavy.tj43}", "tasks": n, "blocks_per_s": nbp / tp,
-                             "batched_blocks_per_s_same_group": batched, "batched_over_plain": batched / (nbp / tp),
-                             "note": "device only; plain: one structure at a time (B = 1 per recursion), same kernels"}
-        log(f"plain path: {res['plain_path']}")
+        return {"group": f"{heavy.tj12},{heavy.tj43}", "tasks": n, "blocks_per_s": nbp / tp,
+                "batched_blocks_per_s_same_group": batched, "batched_over_plain": batched / (nbp / tp),
+                "note": "device only; plain: one structure at a time (B = 1 per recursion), same kernels"}
+
+    if args.plain_tasks > 0:
+        leg(res, "plain_path", plain)
     if args.serve > 0:
-        res["serving"] = serving_stream(args.serve, outdir)
-        log(f"serving: {res['serving']}")
+        leg(res, "serving", lambda: serving_stream(args.serve, outdir))
     from blocksvibe_amd import rns
     tiles = rns.tuned()
     res["rns_tiles"] = {f"wm{t}": sum(1 for v in tiles.values() if v == t) for t in rns.TILES}
     if not args.no_cpu:
-        res["cpu_baseline"] = cpu_baseline(groups)
+        ec = res.get("emission_check", {})
+        per_pass = res["end_to_end"]["bytes_written"] * res["config"]["blocks_per_pass"] / max(blocks, 1)
+        rate = ec.get("host_formatter_gb_per_s")
+        leg(res, "cpu_baseline", lambda: cpu_baseline(groups, json_bytes=per_pass,
+                                                     json_rate=rate * 1e9 if rate else None))
     if not args.keep_files:
         shutil.rmtree(outdir, ignore_errors=True)
     print(json.dumps(res), flush=True)

@@ -15,7 +15,7 @@ _C = {"const int32_t*": P, "int32_t*": P, "const int64_t*": P, "int64_t*": P, "v
       "const int8_t*": P, "int8_t*": P, "const uint32_t*": P, "uint32_t*": P, "const uint16_t*": P,
       "uint16_t*": P, "const double*": P, "double*": P, "int64_t": I64, "int": I32, "size_t": SZ,
       "int*": P, "float*": P, "const float*": P, "char*": P,
-      "uint8_t*": P, "const uint8_t*": P}
+      "uint8_t*": P, "const uint8_t*": P, "void**": P}
 
 
 class B3DError(RuntimeError):
@@ -40,6 +40,12 @@ def declarations():
                 types.append(_C[t])
         out.append((name, ret, types))
     return out
+
+
+def constants():
+    """the integer #define constants of include/b3dhip.h (B3D_...)"""
+    return {m.group(1): int(m.group(2)) for m in re.finditer(r"^#define\s+(B3D_\w+)\s+(-?\d+)\s*$", open(HEADER).read(),
+                                                               flags=re.M)}
 
 
 def lib():

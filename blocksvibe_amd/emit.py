@@ -21,6 +21,7 @@ copies.  Values the GPU formatter leaves to the host (exponent >= 0, or fraction
 dispatcher redo that unit with the library's host formatter (`assemble_host`, byte-identical).
 """
 import concurrent.futures as cf
+import contextlib
 import os
 import queue
 import threading
@@ -28,6 +29,7 @@ import time
 
 import numpy as np
 
+from . import _lib
 from . import b3djson as BJ
 
 UNIT_SLOTS = 1 << 22
@@ -189,8 +191,11 @@ class Emitter:
     the first error.  Thread-safe: several threads may emit, each on its own stream.  unlink: remove each file
     after writing it (throughput runs that must not fill the disk)."""
 
-    def __init__(self, writers=16, buffers=8, unit_slots=UNIT_SLOTS, unlink=False, pending=16):
+    def __init__(self, writers=16, buffers=8, unit_slots=UNIT_SLOTS, unlink=False, pending=16, gate=None):
+        """gate: a context manager factory held around the dispatcher's device work (serve.Server's gate.device,
+        so that no copy or synchronisation of this thread runs beside one of the server's stream captures)"""
         import torch
+        self.gate = gate or contextlib.nullcontext
         self.torch = torch
         self.unit_slots = unit_slots
         self.unlink = unlink
@@ -271,14 +276,16 @@ class Emitter:
             if make:
                 self.made += 1
         if make:
-            return torch.empty(max(nbytes, 1 << 20), dtype=torch.uint8, pin_memory=True)
+            with self.gate():
+                return torch.empty(max(nbytes, 1 << 20), dtype=torch.uint8, pin_memory=True)
         t0 = time.time()
         buf = self.free.get()
         with self.lock:
             self.buffer_wait_s += time.time() - t0
         if buf.numel() < nbytes:
             del buf
-            buf = torch.empty(nbytes, dtype=torch.uint8, pin_memory=True)
+            with self.gate():
+                buf = torch.empty(nbytes, dtype=torch.uint8, pin_memory=True)
         return buf
 
     def _enqueue(self, m2, e1, D, p, eps, jobs, call):
@@ -372,6 +379,11 @@ class Emitter:
         self.q.put(u)
 
     def _dispatch(self):
+        # this thread's HIP calls (page-locked allocations, event waits and synchronisations) run beside stream
+        # captures of other threads (the serving loop's): in the relaxed capture mode they leave those alone
+        if _lib.lib().b3d_thread_capture_mode(2) < 0:
+            with self.lock:
+                self.errors.append(_lib.B3DError(_lib.lib().b3d_last_error().decode()))
         while True:
             u = self.q.get()
             if u is None:
@@ -397,16 +409,17 @@ class Emitter:
         # the copy: a host buffer (waiting here, not in emit(), when the writers are behind), then the bytes
         torch = self.torch
         u.buf = self._buffer(u.ub)
-        u.meta = torch.empty(u.dmeta.numel(), dtype=torch.int64, pin_memory=True)
-        cs = self.copy_stream
-        cs.wait_event(u.ready)
-        with torch.cuda.stream(cs):
-            u.buf[:u.ub].copy_(u.packed, non_blocking=True)
-            u.meta.copy_(u.dmeta, non_blocking=True)
-        done = torch.cuda.Event()
-        done.record(cs)
-        done.synchronize()
-        u.packed = u.dmeta = None
+        with self.gate():
+            u.meta = torch.empty(u.dmeta.numel(), dtype=torch.int64, pin_memory=True)
+            cs = self.copy_stream
+            cs.wait_event(u.ready)
+            with torch.cuda.stream(cs):
+                u.buf[:u.ub].copy_(u.packed, non_blocking=True)
+                u.meta.copy_(u.dmeta, non_blocking=True)
+            done = torch.cuda.Event()
+            done.record(cs)
+            done.synchronize()
+            u.packed = u.dmeta = None
         self.pending.release()
         meta = u.meta.numpy()
         F = len(u.jobs)
@@ -416,10 +429,11 @@ class Emitter:
             self.fallback_units += 1
             torch = self.torch
             allp = np.concatenate([j.polys for j in u.jobs])
-            sel = torch.from_numpy(allp).to(u.e1.device)
             L, D = u.m2.shape[0], u.D
-            m = u.m2.reshape(L, -1, D).index_select(1, sel).cpu().numpy().reshape(L, -1)
-            e = u.e1.reshape(-1, D).index_select(0, sel).cpu().numpy().reshape(-1)
+            with self.gate():
+                sel = torch.from_numpy(allp).to(u.e1.device)
+                m = u.m2.reshape(L, -1, D).index_select(1, sel).cpu().numpy().reshape(L, -1)
+                e = u.e1.reshape(-1, D).index_select(0, sel).cpu().numpy().reshape(-1)
             local, at = [], 0
             for j in u.jobs:
                 local.append(FileJob(j.path, j.head, j.skel, np.arange(at, at + len(j.polys))))

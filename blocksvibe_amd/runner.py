@@ -41,7 +41,7 @@ class Runner:
         from .emit import Emitter
         from .serve import Server
         self.srv = Server(streams=streams, window=window, map_cache=map_cache)
-        self.emitter = Emitter(writers=writers, buffers=4)
+        self.emitter = Emitter(writers=writers, buffers=4, gate=self.srv.gate.device)
         self.lock = threading.Lock()
         self.plans = {}
 
@@ -57,8 +57,11 @@ class Runner:
         """Run one group's tasks (TaskSpec, one group_key) from their setup file; paths relative to cwd.  Blocks
         until every file is written; returns [(error text or None, seconds)] per task."""
         from . import setup as SU
+        from .serve import _relax_thread
+        _relax_thread()                 # this thread uploads inputs beside the server's captures
         t0 = time.time()
-        M, stasks = SU.read(setup_path, L, m_path=m_path)
+        with self.srv.gate.device():
+            M, stasks = SU.read(setup_path, L, m_path=m_path)
         by_q4 = {(st.q4, st.sign): st for st in stasks}
         runs = []
         for t in tasks:
@@ -70,7 +73,8 @@ class Runner:
                 run.error = f"setup file has no structure q4={t.q4} sign={t.sign}"
                 continue
             try:
-                run.req = self._request(t, st, M, L, cwd)
+                with self.srv.gate.device():                          # uploads h_inf: never beside a capture
+                    run.req = self._request(t, st, M, L, cwd)
             except Exception as err:                                  # noqa: BLE001 (reported per task)
                 run.error = f"{type(err).__name__}: {err}"
         for run in runs:

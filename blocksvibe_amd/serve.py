@@ -144,6 +144,21 @@ class _Gate:
                 self.c.notify_all()
 
     @contextlib.contextmanager
+    def device(self):
+        """device work of a helper thread (the file emitter's copies): waits only while a capture runs, not for
+        captures that wait (a worker inside an eager section may be waiting on this very helper)"""
+        with self.c:
+            while self.capturing:
+                self.c.wait()
+            self.active += 1
+        try:
+            yield
+        finally:
+            with self.c:
+                self.active -= 1
+                self.c.notify_all()
+
+    @contextlib.contextmanager
     def capture(self):
         with self.c:
             self.waiting += 1
@@ -386,7 +401,8 @@ class Server:
         t0 = time.time()
         head = batch[0]
         op = self._operator(head.key, head.M)
-        operators = self._operators(batch)
+        with self.gate.eager():                     # device work: never beside a capture
+            operators = self._operators(batch)
         if operators is not None:
             self.mixed_batches += 1
         for r in batch:

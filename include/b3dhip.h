@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-#define B3D_ABI_VERSION 15
+#define B3D_ABI_VERSION 16
 
 /* Last error message of the calling thread (empty if none). */
 const char* b3d_last_error(void);
@@ -244,6 +244,59 @@ int b3d_fmt_pack(const uint8_t* text, const int32_t* len, const uint8_t* last, c
 int b3d_fmt_pack_files(const uint8_t* text, const int32_t* len, const uint8_t* last, const int32_t* pre,
                        const int32_t* post, const uint8_t* strtab, const int64_t* stroff, int64_t nstr,
                        const int64_t* off, int64_t n, uint8_t* out, void* stream);
+
+/* ---- the index plan of the radial recursion (host code; no GPU needed).  Replaces reference
+ *      blocksvibe/recursion_gpu.py:136-365 (Plan, numpy) and its pole bookkeeping structure.py:16-74. ---- */
+
+/* Build the plan of one operator: two_j12, two_j43 (doubled, one parity), two_j_max, radial order, the
+ * pseudo-parities ((pp0_left, pp0_right), (pp1_left, pp1_right)) of the two parities, reuse: recycle physical
+ * slots (LIFO, the reference's numbering) or not.  *plan receives an opaque handle (free with b3d_plan_free).
+ * The tables are read with b3d_plan_get. */
+int b3d_plan_build(int two_j12, int two_j43, int two_j_max, int order, int pp0_left, int pp0_right, int pp1_left,
+                   int pp1_right, int reuse, void** plan);
+void b3d_plan_free(void* plan);
+/* Copy table `field` (of level `level` for the per-level fields, of the given level for the LEVEL_TJS/NPOL/FIRST
+ * fields, ignored otherwise) into out (at most cap int64 entries; out may be null to ask the length).  Returns
+ * the table's length, -1 for a bad field or level.  Plan fields:
+ *   SCALARS      nslots, nphys, SLm, SRm, S2 = SLm SRm, Smax, Jh, dmax, order, b = two_j12 mod 2
+ *   SL, SR       (2, n) structure counts per parity and spin two_j = b + 2 i
+ *   PHYS         physical slot of each of the 2 nslots logical slots (-1: never written)
+ *   MKEYS        per M key in index order: pole type, k, 0 (L) or 1 (R), two_jNN, pseudo-parity, two_j
+ *   QKEYS        per Q key in index order: pole type, k, two_j, sign;   LQPAIRS: per (L, Q) pair: M key, Q key
+ *   LEVEL_TJS, LEVEL_NPOL, LEVEL_FIRST: spins of the level's residue blocks, their pole counts, first slots
+ * Level fields:
+ *   LEVEL        present, m, nb (groups), Kc (columns per group row), nH (Cauchy rows), T (tasks), dmax
+ *   GSRC, GTJP   per group its source parity and pole spin;  NCOL: its prime-pole columns
+ *   COL, COLD    (nb, Kc) physical slot (nphys: zero slot) and doubled position of each column
+ *   RG, RD       per Cauchy row its group and doubled position D'
+ *   T_SLOT, T_HROW, T_L, T_R, T_Q, T_QL: per task its physical slot, Cauchy row, M keys, Q key, (L, Q) pair
+ *   T_KEYS       (6, T): pole type, k, two_j, pseudo-parities (left, right), sign */
+#define B3D_PLAN_SCALARS 0
+#define B3D_PLAN_SL 1
+#define B3D_PLAN_SR 2
+#define B3D_PLAN_PHYS 3
+#define B3D_PLAN_MKEYS 4
+#define B3D_PLAN_QKEYS 5
+#define B3D_PLAN_LQPAIRS 6
+#define B3D_PLAN_LEVEL_TJS 7
+#define B3D_PLAN_LEVEL_NPOL 8
+#define B3D_PLAN_LEVEL_FIRST 9
+#define B3D_PLAN_LEVEL 16
+#define B3D_PLAN_GSRC 17
+#define B3D_PLAN_GTJP 18
+#define B3D_PLAN_NCOL 19
+#define B3D_PLAN_COL 20
+#define B3D_PLAN_COLD 21
+#define B3D_PLAN_RG 22
+#define B3D_PLAN_RD 23
+#define B3D_PLAN_T_SLOT 24
+#define B3D_PLAN_T_HROW 25
+#define B3D_PLAN_T_L 26
+#define B3D_PLAN_T_R 27
+#define B3D_PLAN_T_Q 28
+#define B3D_PLAN_T_QL 29
+#define B3D_PLAN_T_KEYS 30
+int64_t b3d_plan_get(const void* plan, int field, int level, int64_t* out, int64_t cap);
 
 #ifdef __cplusplus
 }

@@ -40,3 +40,21 @@ def test_cauchy_rows_per_lane_checked():
     p = buf.ctypes.data
     with pytest.raises(_lib.B3DError, match="rows_per_lane"):
         _lib.call("b3d_cauchy_level", 34, p, p, 1, 1, p, p, 1, p, p, p, 1, p, p, p, 1, p, p, 1, p, p, p, 1, 3, 0)
+
+
+def test_plan_builder_arguments():
+    """b3d_plan_build (host code) rejects arguments outside its domain with a message, and b3d_plan_get answers
+    -1 for an unknown table or a level out of range; the plan object frees cleanly"""
+    from blocksvibe_amd import _lib
+    h = ctypes.c_void_p()
+    for args in ((2, 1, 10, 5, 0, 0, 1, 1), (2, 2, 10, -1, 0, 0, 1, 1), (2, 2, 10, 5, 0, 2, 1, 1)):
+        with pytest.raises(_lib.B3DError, match="b3d_plan_build"):
+            _lib.call("b3d_plan_build", *args, 1, ctypes.byref(h))
+    _lib.call("b3d_plan_build", 2, 2, 6, 4, 0, 0, 1, 1, 1, ctypes.byref(h))
+    K = _lib.constants()
+    lib = _lib.lib()
+    assert lib.b3d_plan_get(h, K["B3D_PLAN_SCALARS"], 0, None, 0) == 10
+    assert lib.b3d_plan_get(h, 99, 0, None, 0) == -1
+    assert lib.b3d_plan_get(h, K["B3D_PLAN_COL"], 4, None, 0) == -1
+    assert lib.b3d_plan_get(h, K["B3D_PLAN_LEVEL_TJS"], -1, None, 0) == -1
+    lib.b3d_plan_free(h)
