@@ -237,7 +237,11 @@ class Derivs:
         self.nrows = 2 * self.Jo * self.NP
         self.mom = mpf.zeros((self.nrows, self.N, self.R + 1), self.L)
         self.W = window
-        self.stage = mpf.zeros((window, self.nrows, self.N), self.L)
+        # the flush reads a staged entry only when its top limb is nonzero (zero: nothing staged there), so only
+        # that limb plane is cleared, here and after each flush; the other limbs and the exponents of an
+        # entry are written by the stage kernel before they are read
+        self.stage = mpf.empty((window, self.nrows, self.N), self.L)
+        self.stage.m[self.L - 1].zero_()
         self.m0, self.pending = 0, False
 
     def _flush(self):
@@ -245,7 +249,7 @@ class Derivs:
         _lib.call("b3d_moments_flush", self.L, self.KL, P(self.stage.m), P(self.stage.e), self.stage.numel(), self.W,
                   self.m0, P(self.srow), P(self.NK), self.NK.shape[1], P(self.NKb), P(self.mom.m), P(self.mom.e),
                   self.mom.numel(), self.nrows, self.N, self.R + 1, mpf.stream_ptr())
-        self.stage.m.zero_()
+        self.stage.m[self.L - 1].zero_()
         self.pending = False
 
     def accumulate(self, m, res):
