@@ -12,14 +12,17 @@ about equal cost; 10 consecutive steps are one pass over the whole workload.  A 
 batches: the radial recursion (Cauchy + pole steps), derivative moments, numerator GEMMs and the write_output
 GEMM with inputs resident in HBM, then every spin file of the batch made on the GPU (exact decimal text, JSON
 layout), copied to the host and written to local disk by host threads, overlapped with the next batch's
-device work; the step ends when its last file is closed.  `value` = blocks (files closed) / wall time of the
-timed steps.  Files go to a temporary directory and are removed right after they are closed, so a run
+device work.  The batches of the K timed steps go to the lanes (two HIP streams) through one queue, so a lane
+starts the next step's work without waiting for the other at a step boundary (--step-sync: a barrier after
+every step instead); the timed region ends when the last file of the K steps is closed.  `value` = blocks
+(files closed) / wall time of the timed steps.  Files go to a temporary directory and are removed right after they are closed, so a run
 does not fill the disk (the write into the page cache is counted; like blocks_3d, nothing is fsync'ed).
 
 Beside it: the device-only rate (one pass without the files), the per-operator constant setup time, the
 plain per-task path (B = 1), a mixed serving stream (p50/p99 to files written) and a CPU baseline.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--serve SECONDS] [--no-cpu] [--keep-files DIR]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--serve SECONDS] [--no-cpu] [--step-sync]
+                    [--keep-files DIR]
 """
 import argparse
 import json
@@ -54,6 +57,8 @@ def parse():
     ap.add_argument("--serve", type=float, default=20.0, help="seconds of the mixed serving stream (0: skip)")
     ap.add_argument("--writers", type=int, default=16)
     ap.add_argument("--streams", type=int, default=2, help="lanes of operator groups per step, one HIP stream each")
+    ap.add_argument("--step-sync", action="store_true",
+                    help="synchronise the lanes at every step boundary (default: the K steps' batches in one queue)")
     ap.add_argument("--keep-files", default=None, help="write the files under this directory and keep them")
     ap.add_argument("--json-out", default=None)
     return ap.parse_args()
@@ -209,32 +214,41 @@ def make_bins(batches, nbins):
 _STREAMS = []
 
 
-def step(bins, i, emitter, timer=None, streams=1):
-    """one step: the chain and the spin files of bin i % len(bins); returns its blocks.  streams > 1: the
-    step's batches run in that many lanes, each lane on its own HIP stream driven by its own thread, so that one
-    lane's small levels and file emission share the GPU with another lane's chain"""
+def step(bins, i, emitter, timer=None, streams=1, count=1, progress=None):
+    """count steps from step i: the chain and the spin files of bins i % len(bins) .. (i + count - 1) %
+    len(bins); returns their blocks.  streams > 1: the batches run in that many lanes, each lane on its own HIP
+    stream driven by its own thread, so that one lane's small levels and file emission share the GPU with
+    another lane's chain; the lanes take the batches of all count steps from one queue (step by step, largest
+    first within a step), so no lane waits for the other at a step boundary.  progress(done batches, total) is
+    called as batches finish."""
     import queue
     import threading
     import torch
-    batches = bins[i % len(bins)]
+    batches = [b for k in range(i, i + count) for b in sorted(bins[k % len(bins)], key=lambda x: -x.cost())]
     nl = min(streams, len(batches))
+    done = [0]
+    done_lock = threading.Lock()
 
     def run_one(b):
         r = b.run(timer=timer)
         if emitter is not None:
             b.emit(emitter, r)
         del r
+        if progress is not None:
+            with done_lock:
+                done[0] += 1
+                progress(done[0], len(batches))
 
     if nl <= 1:
         for b in batches:
             run_one(b)
     else:
-        # the lanes take the step's batches from one queue, largest first, so that both streams stay busy
-        # until the last batch has started (a fixed split leaves one idle when the costs are uneven)
+        # the lanes take the batches from one queue, so that both streams stay busy until the last batch has
+        # started (a fixed split leaves one idle when the costs are uneven)
         while len(_STREAMS) < nl:
             _STREAMS.append(torch.cuda.Stream())
         work = queue.Queue()
-        for b in sorted(batches, key=lambda x: -x.cost()):
+        for b in batches:
             work.put(b)
         main = torch.cuda.current_stream()
         errors = []
@@ -636,11 +650,20 @@ def main():
     per_step = []
     files0, bytes0, write0, wait0 = emitter.files, emitter.bytes, emitter.write_s, emitter.wait_s
     t0 = time.time()
-    for s in range(args.steps):
-        ts = time.time()
-        blocks += step(bins, args.warmup + s, emitter, streams=args.streams)
-        per_step.append(time.time() - ts)
-        log(f"step {s} done at {time.time() - t0:.1f} s ({per_step[-1]:.2f} s)")
+    if args.step_sync:
+        for s in range(args.steps):
+            ts = time.time()
+            blocks += step(bins, args.warmup + s, emitter, streams=args.streams)
+            per_step.append(time.time() - ts)
+            log(f"step {s} done at {time.time() - t0:.1f} s ({per_step[-1]:.2f} s)")
+    else:
+        # the K steps' batches in one queue: the lanes go from one step's work to the next without waiting
+        # for each other at the step boundaries; the region still holds exactly K steps of work
+        nb_total = sum(len(bins[(args.warmup + s) % len(bins)]) for s in range(args.steps))
+        report = max(1, nb_total // 10)
+        blocks = step(bins, args.warmup, emitter, streams=args.streams, count=args.steps,
+                      progress=lambda d, n: d % report == 0 and log(f"{d}/{n} batches done at "
+                                                                     f"{time.time() - t0:.1f} s"))
     torch.cuda.synchronize()
     dt = time.time() - t0
     res = {"metric": METRIC, "value": blocks / dt, "unit": "blocks/s", "n_gpus": 1, "steps": args.steps,
@@ -654,8 +677,10 @@ def main():
                       "step": "chain + GPU JSON text + copy + spin files written and closed"},
            "end_to_end": {"files": emitter.files - files0, "bytes_written": emitter.bytes - bytes0,
                           "gb_per_s_written": (emitter.bytes - bytes0) / 1e9 / max(dt, 1e-9),
-                          "host_fallback_units": emitter.fallback_units, "step_s_min": min(per_step),
-                          "step_s_max": max(per_step), "constants_setup_s": setup_s,
+                          "host_fallback_units": emitter.fallback_units,
+                          "step_s_min": min(per_step) if per_step else None,
+                          "step_s_max": max(per_step) if per_step else None,
+                          "steps_pipelined": not args.step_sync, "constants_setup_s": setup_s,
                           "writer_busy_s": emitter.write_s - write0, "writers": args.writers,
                           "streams": args.streams,
                           "chain_waited_for_writers_s": emitter.wait_s - wait0, "out_dir": outdir,
