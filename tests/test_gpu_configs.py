@@ -18,7 +18,8 @@ tests/test_blocks_recursion.py:12-52, tests/test_dropin.py:20-59):
         and in the mix
   cfg5  four spin-2 externals at (8, 8), spins 0..64, order 90, nmax 10, kept 20, 1536 bits (L = 50), xt_radial:
         one right-hand side through the whole chain, identical with and without slot recycling and alone or in a
-        batch of two
+        batch of two; and the whole chain of a batch of two captured as one HIP graph, every replay identical
+        to the eager run of its input
 Operators and h_inf are seeded synthetic data of the real shapes (blocks_3d's own M and h_inf need its sources).
 """
 import json
@@ -310,3 +311,40 @@ def test_cfg5_large_rhs_properties():
         b = outs[k]
         assert torch.equal(a[0], b[0]) and torch.equal(a[1], b[1]), k
     assert (a[1] != R.EZ).any()
+
+
+def test_cfg5_chain_captured_in_one_graph():
+    """cfg5's whole chain for a batch of two right-hand sides (recursion with the moment hook, numerators,
+    write_output) captured as one HIP graph (the serving loop's capture: thread-local mode, a side stream) and
+    replayed with its static input refilled: every replay bit-identical to the eager run of the same input,
+    with another input in between"""
+    from blocksvibe_amd import mpf
+    from blocksvibe_amd.derivatives import Derivs
+    from blocksvibe_amd.output import Output
+    from blocksvibe_amd.plan import Plan
+    from blocksvibe_amd.recursion import Recursion
+    from blocksvibe_amd.serve import _Graph
+    L, order, tjmax, Lam, kept, pp = 50, 90, 128, 19, 20, ((0, 0), (1, 1))
+    plan = Plan(8, 8, tjmax, order, pp, reuse=True)
+    rec = Recursion(plan, None, L, seed=11)
+    der = Derivs(plan, Lam, L, kept_pole_order=kept)
+    out = Output("xt_radial", Lam, 0, "0", L)
+
+    def chain(hs):
+        h = hs[0]
+        dn = der.for_order(Lam)
+        dn.start(h.shape[-1])
+        rec.run(h, hook=dn.accumulate)
+        return out.apply_raw([dn.finish(h)])
+
+    host = lambda r: (r.m.cpu(), r.e.cpu())
+    ha = mpf.random_device((2, order + 1, plan.Jh, plan.S2, 2), L, seed=21)
+    hb = mpf.random_device((2, order + 1, plan.Jh, plan.S2, 2), L, seed=22)
+    want_a, want_b = host(chain([ha])), host(chain([hb]))     # eager: builds every lazily made table
+    torch.cuda.synchronize()
+    g = _Graph(chain, [ha])
+    print(f"cfg5 graph: {plan.nphys} physical slots, pool {g.bytes / 2**30:.2f} GiB", flush=True)
+    for h, want in ((ha, want_a), (hb, want_b), (ha, want_a)):
+        got = host(g.replay([h]))
+        assert torch.equal(got[0], want[0]) and torch.equal(got[1], want[1])
+    assert not torch.equal(want_a[0], want_b[0])
