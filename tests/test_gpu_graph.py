@@ -1,7 +1,8 @@
 """Captured request pipelines in the serving loop (blocksvibe_amd.serve): a batch shape that has run once is
 captured as a HIP graph the next time and replayed afterwards.  Every replay must be bit-identical to the
 eager result of the same inputs, including replays with new inputs, with eager work of another operator and
-device synchronisations between the replays."""
+device synchronisations between the replays; and the chains of tasks of different limb counts, orders, nmax and
+spin ranges captured together as one graph."""
 import pytest
 import torch
 
@@ -90,3 +91,44 @@ def test_capture_beside_synchronous_copies():
     assert srv.capture_failures == 0, srv.capture_failures
     assert srv.replays >= 2, srv.replays
     assert all(_same(g, want) for g in got)
+
+
+def test_uneven_tasks_in_one_graph():
+    """tasks of different limb counts, orders, nmax and spin ranges captured together as one HIP graph (one
+    launch replays the chains of all of them): every replay identical to the eager runs of its inputs"""
+    from blocksvibe_amd import mpf
+    from blocksvibe_amd.derivatives import Derivs
+    from blocksvibe_amd.output import Output
+    from blocksvibe_amd.plan import Plan
+    from blocksvibe_amd.recursion import Recursion
+    from blocksvibe_amd.serve import _Graph
+    pp = ((0, 0), (1, 1))
+    tasks = []
+    for i, (tj12, tj43, tjmax, order, L, Lam, kept, B) in enumerate(
+            [(2, 2, 6, 10, 10, 5, 6, 2), (4, 2, 12, 20, 18, 7, 8, 1), (0, 0, 16, 14, 26, 3, 4, 3)]):
+        plan = Plan(tj12, tj43, tjmax, order, pp)
+        tasks.append((plan, Recursion(plan, None, L, seed=40 + i), Derivs(plan, Lam, L, kept_pole_order=kept),
+                      Output("xt_radial", Lam, 0, "1.5", L), Lam, L, B))
+
+    def chains(hs):
+        outs = []
+        for (plan, rec, der, out, Lam, L, B), h in zip(tasks, hs):
+            dn = der.for_order(Lam)
+            dn.start(B)
+            rec.run(h, hook=dn.accumulate)
+            outs.append(out.apply_raw([dn.finish(h)]))
+        return outs
+
+    def inputs(seed):
+        return [mpf.random_device((2, plan.order + 1, plan.Jh, plan.S2, B), L, seed=seed + i)
+                for i, (plan, _, _, _, _, L, B) in enumerate(tasks)]
+
+    host = lambda rs: [(r.m.cpu(), r.e.cpu()) for r in rs]
+    same = lambda a, b: all(torch.equal(x[0], y[0]) and torch.equal(x[1], y[1]) for x, y in zip(a, b))
+    ha, hb = inputs(500), inputs(600)
+    want_a, want_b = host(chains(ha)), host(chains(hb))
+    torch.cuda.synchronize()
+    g = _Graph(chains, ha)
+    for h, want in ((ha, want_a), (hb, want_b), (ha, want_a)):
+        assert same(host(g.replay(h)), want)
+    assert not same(want_a, want_b)
