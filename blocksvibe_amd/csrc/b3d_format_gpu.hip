@@ -90,45 +90,6 @@ __global__ __launch_bounds__(256) void k_fmt_sizes(const int32_t* __restrict__ m
   }
 }
 
-// The same sizes with one wave per polynomial (deg <= 64): lane d loads coefficient d's exponent and top limb
-// (consecutive entries: one coalesced load each for the wave), then the scan from the top coefficient down runs
-// on wave-uniform values taken from the lanes (readlane); the rare comparison that needs whole mantissas loads
-// them as before.  The per-thread form above scans its polynomial's entries one dependent load at a time.
-template <int L>
-__global__ __launch_bounds__(256) void k_fmt_sizes_w(const int32_t* __restrict__ m, const i64* __restrict__ e, i64 ps,
-                                                     const int64_t* __restrict__ pbase, i64 npoly, int deg, i64 eps,
-                                                     int32_t* __restrict__ sizes) {
-  const int lane = threadIdx.x & 63;
-  for (i64 q = blockIdx.x * (i64)(blockDim.x >> 6) + (threadIdx.x >> 6); q < npoly;
-       q += (i64)gridDim.x * (blockDim.x >> 6)) {
-    const i64 base = pbase[q];
-    i64 el = B3D_EZ;
-    u32 tl = 0u;
-    if (lane < deg) {
-      el = e[base + lane];
-      tl = (u32)m[(i64)(L - 1) * ps + base + lane];
-    }
-    const u32 elo = (u32)(u64)el, ehi = (u32)((u64)el >> 32);
-    int size = 0;
-    i64 mx = -1, emx = 0;
-    u32 tmx = 0u;
-    for (int d = deg - 1; d >= 0; --d) {
-      const i64 ei = (i64)(((u64)(u32)__builtin_amdgcn_readlane((int)ehi, d) << 32) |
-                           (u32)__builtin_amdgcn_readlane((int)elo, d));
-      if (ei == B3D_EZ) continue;   // canonical zero: 0 < 2^eps 0 is false
-      const u32 ti = (u32)__builtin_amdgcn_readlane((int)tl, d);
-      const i64 i = base + d;
-      if (mx < 0 || lt_abs_fast<L>(m, ps, mx, tmx, emx, i, ti, ei + eps)) size = d + 1;
-      if (mx < 0 || lt_abs_fast<L>(m, ps, mx, tmx, emx, i, ti, ei)) {
-        mx = i;
-        emx = ei;
-        tmx = ti;
-      }
-    }
-    if (lane == 0) sizes[q] = size;
-  }
-}
-
 template <int L>
 __global__ __launch_bounds__(256) void k_fmt_text(const int32_t* __restrict__ m, const i64* __restrict__ e, i64 ps,
                                                   const int64_t* __restrict__ ent, i64 n, int p, uint8_t* __restrict__ dg,
@@ -425,11 +386,7 @@ int b3d_fmt_sizes(int L, const int32_t* m, const int64_t* e, int64_t ps, const i
   }
   if (b3d::check_planes("b3d_fmt_sizes", {ps})) return 2;
   hipStream_t s = (hipStream_t)stream;
-  if (deg <= 64) {
-    B3D_DISPATCH_L(L, k_fmt_sizes_w<LL><<<dim3(grid_for(npoly, 4)), dim3(256), 0, s>>>(m, e, ps, pbase, npoly, deg, eps_exp, sizes));
-  } else {
-    B3D_DISPATCH_L(L, k_fmt_sizes<LL><<<dim3(grid_for(npoly, 256)), dim3(256), 0, s>>>(m, e, ps, pbase, npoly, deg, eps_exp, sizes));
-  }
+  B3D_DISPATCH_L(L, k_fmt_sizes<LL><<<dim3(grid_for(npoly, 256)), dim3(256), 0, s>>>(m, e, ps, pbase, npoly, deg, eps_exp, sizes));
   return b3d::check_launch("b3d_fmt_sizes");
 }
 
