@@ -312,7 +312,25 @@ def roofline(timer):
     return {"kernel": "Cauchy step: k_cauchy_pairs<34> / k_cauchy_groups<34>", "bound": "hbm", "achieved": achieved,
             "peak": 8000.0, "unit": "GB/s", "frac": achieved / 8000.0, "traffic": pmc_traffic(WORKLOAD),
             "launches": len(timer), "avg_launch_ms": avg_s * 1e3, "algorithmic_bytes_per_launch": per_launch,
-            "note": "VALU-bound (one-limb long divisions): see profiles/ for VALU issue utilisation"}
+            "note": "VALU-bound (one-limb long divisions), not HBM-bound: valu_issue gives the limit that applies",
+            "valu_issue": valu_issue()}
+
+
+COUNTERS_SUMMARY = os.path.join(ROOT, "profiles", "heavy_group_counters.txt")
+
+
+def valu_issue():
+    """the Cauchy kernels' VALU issue utilisation (SQ_INSTS_VALU per launch over the 614.4 G wave-instructions/s
+    peak) from the committed rocprofv3 counters of the heaviest operator group, or None without them"""
+    out = {}
+    try:
+        for line in open(COUNTERS_SUMMARY):
+            f = line.split()
+            if f and f[0] in ("k_cauchy_pairs<34>", "k_cauchy_groups<34>") and f[4].endswith("%"):
+                out[f[0]] = float(f[4].rstrip("%")) / 100.0
+    except (OSError, IndexError, ValueError):
+        return None
+    return {"per_kernel": out, "source": "profiles/heavy_group_counters.txt"} if out else None
 
 
 def _random_pairs(rng, n, L, spread):
