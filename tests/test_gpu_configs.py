@@ -231,7 +231,7 @@ def test_cfg3_heavy_group_bit_exact():
             worst = max(worst, err)
         bits = float(-mp.log(worst, 2)) if worst else float("inf")
         print(f"cfg3 numerators spin {tj / 2:g} parity {p}: within 2^-{bits:.1f} of the largest coefficient", flush=True)
-        # the north star's bound for 1024-bit blocks: 2^-(precision - 32)
+        # the required agreement for 1024-bit blocks: 2^-(precision - 32)
         assert bits > 1024 - 32, (tj, p, bits)
 
 
