@@ -38,6 +38,7 @@ sys.path.insert(0, ROOT)
 ORDER, LAMBDA, KEPT, PREC, TJMAX, D12 = 60, 11, 14, 1024, 60, "6"
 TWO_JS = (4, 4, 4, 4)
 NQ_PER_SIGN = 11
+ORDERS_PER_RUN = 1       # lambda orders per recursion (0: all of a sign's orders in one)
 BINS = 10
 PP = ((0, 0), (1, 1))
 WORKLOAD = ("nmax=6 spin-2 externals: 25 operator groups x 22 structures x 6 lambda orders, order 60, kept 14, "
@@ -61,6 +62,8 @@ def parse():
                     help="synchronise the lanes at every step boundary (default: the K steps' batches in one queue)")
     ap.add_argument("--keep-files", default=None, help="write the files under this directory and keep them")
     ap.add_argument("--json-out", default=None)
+    ap.add_argument("--orders-per-run", type=int, default=1,
+                    help="lambda orders whose right-hand sides share one recursion (0: all of a sign's orders)")
     return ap.parse_args()
 
 
@@ -112,8 +115,11 @@ class Batch:
         pl = group.plan
         self.out = output_map(self.p0, group.L)
         self.ns = list(range(self.p0, LAMBDA + 1, 2))
-        self.hs = [mpf.random_device((2, ORDER + 1, pl.Jh, pl.S2, NQ_PER_SIGN), group.L,
-                                     seed=seed * 100 + 10 * self.p0 + i) for i in range(len(self.ns))]
+        hs = [mpf.random_device((2, ORDER + 1, pl.Jh, pl.S2, NQ_PER_SIGN), group.L,
+                                seed=seed * 100 + 10 * self.p0 + i) for i in range(len(self.ns))]
+        # resident with the lambda orders side by side (one recursion takes the right-hand sides of all of them)
+        self.h_all = mpf.cat(hs, 4)
+        self.hs = [self.h_all[..., i * NQ_PER_SIGN:(i + 1) * NQ_PER_SIGN] for i in range(len(self.ns))]
         self.tasks = []
         for qi, q4 in enumerate(structures(sign)):
             argv = ["--j-external", ",".join(f"{x / 2:g}" for x in TWO_JS), "--j-internal", f"0-{TJMAX // 2}",
@@ -145,13 +151,17 @@ class Batch:
         return pl.S2 * (pl.cauchy_terms() + pole) * len(self.ns)
 
     def run(self, timer=None):
-        """the chain -> write_output result (outputs, Jo, 2, S2, nq, Dmax), on the device"""
+        """the chain -> write_output result (outputs, Jo, 2, S2, nq, Dmax), on the device: the recursion over
+        the right-hand sides of ORDERS_PER_RUN lambda orders at a time (all of them by default; one recursion,
+        each order's moments and numerators from its own columns)"""
+        from blocksvibe_amd import derivatives as DV
+        k = ORDERS_PER_RUN or len(self.ns)
         nums = []
-        for n, h in zip(self.ns, self.hs):
-            dn = self.der.for_order(LAMBDA - n)
-            dn.start(h.shape[-1])
-            self.g.rec.run(h, hook=dn.accumulate, timer=timer)
-            nums.append(dn.finish(h))
+        for i0 in range(0, len(self.ns), k):
+            ns, hs = self.ns[i0:i0 + k], self.hs[i0:i0 + k]
+            whole = len(ns) == len(self.ns)
+            nums += DV.run_orders(self.g.rec, self.der, hs, [LAMBDA - n for n in ns], timer=timer,
+                                  h_all=self.h_all if whole else None)
         return self.out.apply_raw(nums)
 
     def jobs(self):
@@ -641,6 +651,8 @@ def fs_type(path):
 
 def main():
     args = parse()
+    global ORDERS_PER_RUN
+    ORDERS_PER_RUN = args.orders_per_run
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
@@ -740,6 +752,12 @@ def main():
     from blocksvibe_amd import rns
     tiles = rns.tuned()
     res["rns_tiles"] = {f"wm{t}": sum(1 for v in tiles.values() if v == t) for t in rns.TILES}
+    ms = torch.cuda.memory_stats()
+    res["device_memory"] = {"peak_reserved_gb": ms.get("reserved_bytes.all.peak", 0) / 1e9,
+                            "peak_allocated_gb": ms.get("allocated_bytes.all.peak", 0) / 1e9,
+                            "alloc_retries": ms.get("num_alloc_retries", 0),
+                            "device_allocs": ms.get("num_device_alloc", 0), "device_frees": ms.get("num_device_free", 0),
+                            "orders_per_run": ORDERS_PER_RUN}
     if not args.no_cpu:
         ec = res.get("emission_check", {})
         per_pass = res["end_to_end"]["bytes_written"] * res["config"]["blocks_per_pass"] / max(blocks, 1)

@@ -49,10 +49,16 @@ __global__ __launch_bounds__(256) void k_mom_flush(const int32_t* __restrict__ s
                                                    int W, int m0, const int32_t* __restrict__ srow,
                                                    const uint32_t* __restrict__ NK, i64 nkps, int KL,
                                                    const int32_t* __restrict__ NKb, int32_t* __restrict__ vm,
-                                                   i64* __restrict__ ve, i64 vps, i64 nrows, i64 N, int R1) {
-  for (i64 t = blockIdx.x * (i64)blockDim.x + threadIdx.x; t < nrows * N * R1; t += (i64)gridDim.x * blockDim.x) {
-    const int k = (int)(t % R1);
-    const i64 c = (t / R1) % N, row = t / ((i64)R1 * N);
+                                                   i64* __restrict__ ve, i64 vps, i64 nrows, i64 N, int R1, i64 B,
+                                                   const int32_t* __restrict__ jk, int Kc) {
+  // thread (row, structure entry s2, j): moment k of right-hand side b, b R1 + k = jk[j] (or j without a list)
+  const i64 S2 = N / B;
+  for (i64 t = blockIdx.x * (i64)blockDim.x + threadIdx.x; t < nrows * S2 * Kc; t += (i64)gridDim.x * blockDim.x) {
+    const int j = (int)(t % Kc);
+    const i64 rs = t / Kc;
+    const int code = jk ? jk[j] : j;
+    const int k = code % R1;
+    const i64 c = (rs % S2) * B + code / R1, row = rs / S2;
     u32 r[L];
     const i64 er = mp::mom_window<L, KLM>(sm, se, sps, W, nrows, N, row, c, m0 + srow[row], k, R1, NK, nkps, KL, NKb, r);
     if (r[L - 1] == 0u) continue;
@@ -81,17 +87,26 @@ int b3d_moments_stage(int L, const int32_t* rm, const int64_t* re, int64_t rps, 
 
 int b3d_moments_flush(int L, int KL, const int32_t* sm, const int64_t* se, int64_t sps, int W, int m0,
                       const int32_t* srow, const uint32_t* NK, int64_t nkps, const int32_t* NKb, int32_t* vm,
-                      int64_t* ve, int64_t vps, int64_t nrows, int64_t N, int R1, void* stream) {
+                      int64_t* ve, int64_t vps, int64_t nrows, int64_t N, int R1, int64_t B, const int32_t* jk,
+                      int Kc, void* stream) {
   if (nrows <= 0 || N <= 0 || R1 <= 0) return 0;
+  if (B <= 0 || N % B != 0) {
+    b3d::set_error("b3d_moments_flush: N must be a multiple of B > 0");
+    return 2;
+  }
+  if (!jk) Kc = (int)(B * R1);
+  if (Kc <= 0) return 0;
   if (b3d::check_planes("b3d_moments_flush", {sps, vps})) return 2;
   hipStream_t s = (hipStream_t)stream;
-  const int64_t n = nrows * N * R1;
+  const int64_t n = nrows * (N / B) * Kc;
   if (KL <= 4) {
     B3D_DISPATCH_L(L, k_mom_flush<LL, 4><<<grid_for(n, 256), 256, 0, s>>>(sm, se, sps, W, m0, srow, NK, nkps, KL,
-                                                                           NKb, vm, ve, vps, nrows, N, R1));
+                                                                           NKb, vm, ve, vps, nrows, N, R1, B, jk,
+                                                                           Kc));
   } else if (KL <= 8) {
     B3D_DISPATCH_L(L, k_mom_flush<LL, 8><<<grid_for(n, 256), 256, 0, s>>>(sm, se, sps, W, m0, srow, NK, nkps, KL,
-                                                                           NKb, vm, ve, vps, nrows, N, R1));
+                                                                           NKb, vm, ve, vps, nrows, N, R1, B, jk,
+                                                                           Kc));
   } else {
     b3d::set_error("b3d_moments_flush: n^k needs more than 8 limbs");
     return 2;

@@ -130,6 +130,7 @@ class Derivs:
         self._constants()
         self._finish_consts()
         self._orders = {R: self}
+        self._jk = {}                 # moment-flush column lists per r1 pattern (shared by every order's copy)
 
     # --------------------------------------------------------------------------------------- bookkeeping
     def _levels(self):
@@ -231,11 +232,24 @@ class Derivs:
                            for lv in self.levels]
 
     # --------------------------------------------------------------------------------------- moments
-    def start(self, B, window=8):
-        """zero the moment accumulators for B right-hand sides"""
+    def start(self, B, window=8, r1=None):
+        """zero the moment accumulators for B right-hand sides.  r1: per right-hand side the number of moments
+        k it needs (at most R + 1; right-hand sides of several lambda orders in one recursion, each with its own
+        R): the flushes compute only those (split then hands each order its columns); None: all R + 1"""
         self.B, self.N = B, self.plan.S2 * B
         self.nrows = 2 * self.Jo * self.NP
         self.mom = mpf.zeros((self.nrows, self.N, self.R + 1), self.L)
+        self.jk, self.Kc = None, B * (self.R + 1)
+        if r1 is not None:
+            # built on the first use of a pattern (eagerly: a graph capture of the same shape comes later) and
+            # kept, so that a captured run reads the same list
+            key = (self.R, tuple(r1))
+            jk = self._jk.get(key)
+            if jk is None:
+                assert len(r1) == B and all(0 < x <= self.R + 1 for x in r1)
+                codes = [b * (self.R + 1) + k for b in range(B) for k in range(r1[b])]
+                jk = self._jk.setdefault(key, torch.tensor(codes, dtype=torch.int32, device=self.mom.e.device))
+            self.jk, self.Kc = jk, int(jk.numel())
         self.W = window
         # the flush reads a staged entry only when its top limb is nonzero (zero: nothing staged there), so only
         # that limb plane is cleared, here and after each flush; the other limbs and the exponents of an
@@ -248,7 +262,8 @@ class Derivs:
         P = lambda x: x.data_ptr()
         _lib.call("b3d_moments_flush", self.L, self.KL, P(self.stage.m), P(self.stage.e), self.stage.numel(), self.W,
                   self.m0, P(self.srow), P(self.NK), self.NK.shape[1], P(self.NKb), P(self.mom.m), P(self.mom.e),
-                  self.mom.numel(), self.nrows, self.N, self.R + 1, mpf.stream_ptr())
+                  self.mom.numel(), self.nrows, self.N, self.R + 1, self.B, P(self.jk) if self.jk is not None else 0,
+                  self.Kc, mpf.stream_ptr())
         self.stage.m[self.L - 1].zero_()
         self.pending = False
 
@@ -269,6 +284,28 @@ class Derivs:
                   self.stage.numel(), w * self.nrows * self.N, mpf.stream_ptr())
         if w == self.W - 1:
             self._flush()
+
+    def split(self, parts):
+        """after one recursion over the right-hand sides of several lambda orders (start(B, r1=...)): parts =
+        [(the order's Derivs (for_order(R_i)), first right-hand side b0, count Bi)] -> those Derivs, each holding
+        the moments k <= R_i of its own columns b0 .. b0 + Bi, ready for finish (h_inf: the same columns).  The
+        moments are exactly the ones the order's own run would have accumulated (every kernel treats columns
+        independently)."""
+        if self.pending:
+            self._flush()
+        self.stage = None
+        S2, R1 = self.plan.S2, self.R + 1
+        m4 = self.mom.reshape(self.nrows, S2, self.B, R1)
+        out = []
+        for d, b0, Bi in parts:
+            assert d.R <= self.R and d.plan is self.plan and 0 <= b0 and b0 + Bi <= self.B
+            sub = m4[:, :, b0:b0 + Bi, :d.R + 1].contiguous()
+            d.B, d.N, d.nrows = Bi, S2 * Bi, self.nrows
+            d.mom = sub.reshape(self.nrows, S2 * Bi, d.R + 1)
+            d.stage, d.pending, d.jk, d.Kc = None, False, None, Bi * (d.R + 1)
+            out.append(d)
+        self.mom = None
+        return out
 
     # --------------------------------------------------------------------------------------- final maps
     def finish(self, h_inf, chunk=None):
@@ -399,3 +436,26 @@ class Derivs:
         d.NKb = torch.tensor([x.bit_length() for x in v], dtype=torch.int32, device=mpf.dev())
         d._finish_consts()
         return d
+
+
+def run_orders(rec, der, hs, Rs, operators=None, timer=None, h_all=None, window=8):
+    """the recursion and the derivative stages for the right-hand sides of several lambda orders of one operator
+    and sign in ONE recursion: hs[i] is order i's h_inf MPF (2, order + 1, Jh, S2, B_i), Rs[i] its number of
+    log r derivatives R_i (der.for_order(R_i) holds its constants).  The columns of all orders run through
+    each level together (one launch per level for all of them), the moment flushes compute each column's
+    moments k <= R_i only, and each order's numerators come from its own columns (Derivs.split, finish): the
+    same bits as one recursion per order.  h_all: the hs concatenated along the right-hand sides, if the caller
+    keeps them so; window: levels per moment flush (as Derivs.start).  Returns the numerators per order (as
+    Derivs.finish)."""
+    Bs = [h.shape[-1] for h in hs]
+    if h_all is None:
+        h_all = hs[0] if len(hs) == 1 else mpf.cat(hs, len(hs[0].shape) - 1)
+    dm = der.for_order(max(Rs))
+    dm.start(sum(Bs), window=window, r1=[R + 1 for R, B in zip(Rs, Bs) for _ in range(B)])
+    rec.run(h_all, hook=dm.accumulate, operators=operators, timer=timer)
+    parts, b0 = [], 0
+    for R, B in zip(Rs, Bs):
+        parts.append((der.for_order(R), b0, B))
+        b0 += B
+    return [d.finish(h_all[..., b0:b0 + B]) for d, (_, b0, B) in zip(dm.split(parts), parts)]
+

@@ -3,9 +3,10 @@
 Requests (one blocks_3d task each: operator, four-point sign, lambda orders, h_inf) arrive at any time.
 Worker threads, one per HIP stream, repeatedly take the oldest waiting request and every waiting request
 compatible with it (same operator shape -- plan, limb count, Lambda, kept pole order -- sign and lambda
-orders, any M matrices), up to what the HBM budget still admits, and run them as one batch: one recursion per
-lambda order over all their structures (the pole step reads each right-hand side's own operator tables,
-recursion.Operators), the moments, the numerator GEMMs and the write_output GEMM.  Requests that arrive while batches are on the GPU join the
+orders, any M matrices), up to what the HBM budget still admits, and run them as one batch: one recursion over
+all their structures and lambda orders (the pole step reads each right-hand side's own operator tables,
+recursion.Operators; each order's moments up to its own derivative count, derivatives.run_orders), the
+numerator GEMMs per order and the write_output GEMM.  Requests that arrive while batches are on the GPU join the
 next batch as soon as a worker and memory are free, and batches on different streams overlap on the
 device.  Plans, uploaded level tables, operator constants (M, qL, R tables) and write_output maps are cached
 on the GPU across requests.  A batch shape (operator, sign, batch size, coordinates) that has run once is
@@ -32,7 +33,7 @@ import time
 import torch
 
 from . import _lib, mpf
-from .derivatives import Derivs
+from .derivatives import Derivs, run_orders
 from .output import cached as output_cached
 from .plan import Plan
 from .recursion import Operators, Recursion
@@ -302,7 +303,8 @@ class Server:
                     continue
                 p0 = (1 - head.sign) // 2
                 with self.gate.eager(), self.setup_lock:    # may build derivative constants
-                    per = op.per_rhs_bytes(p0) + op.h_bytes() * len(head.hs)
+                    # the recursion takes each request's right-hand sides once per lambda order
+                    per = (op.per_rhs_bytes(p0) + op.h_bytes()) * len(head.hs)
                 room = self.budget - self.in_use
                 if per <= room or self.in_use == 0:
                     batch = [head]
@@ -366,35 +368,32 @@ class Server:
 
     def _operators(self, batch):
         """recursion.Operators of a batch whose requests belong to more than one operator (None otherwise),
-        cached per assignment of operators to columns (a captured batch keeps reading the same tables)"""
+        cached per assignment of operators to columns (a captured batch keeps reading the same tables).  The
+        recursion takes the batch's right-hand sides once per lambda order (derivatives.run_orders), so the
+        assignment repeats per order."""
         keys = [r.key for r in batch]
         if all(k == keys[0] for k in keys):
             return None
+        nh = len(batch[0].hs)
         with self.setup_lock:
-            ops = self.stacked.get(tuple(keys))
+            ops = self.stacked.get((tuple(keys), nh))
             if ops is None:
                 order = list(dict.fromkeys(keys))
                 ops = Operators([self.ops[k].rec for k in order],
-                                [order.index(k) for r in batch for k in [r.key] * r.hs[0].shape[-1]])
-                self.stacked[tuple(keys)] = ops
+                                [order.index(k) for r in batch for k in [r.key] * r.hs[0].shape[-1]] * nh)
+                self.stacked[(tuple(keys), nh)] = ops
                 while len(self.stacked) > 64:
                     self.stacked.pop(next(iter(self.stacked)))
             return ops
 
     def _pipeline(self, op, key, sign, coords, hs, operators=None):
-        """device work of one batch: per lambda order a recursion over all right-hand sides with the moment
-        hook (operators: the columns' own operators when they differ), the numerators, then the write_output
-        map per (coordinate, Delta_1+2)"""
+        """device work of one batch: one recursion over the right-hand sides of all its lambda orders with the
+        moment hook (derivatives.run_orders; operators: the columns' own operators when they differ), each
+        order's numerators, then the write_output map per (coordinate, Delta_1+2)"""
         tj12, tj43, tjmax, order, Lam, kept, L, pp, op_id = key
         p0 = (1 - sign) // 2
         der = op.derivs_for(p0)
-        B = hs[0].shape[-1]
-        nums = []
-        for i, h in enumerate(hs):
-            dn = der.for_order(Lam - (p0 + 2 * i))
-            dn.start(B)
-            op.rec.run(h, hook=dn.accumulate, operators=operators)
-            nums.append(dn.finish(h))
+        nums = run_orders(op.rec, der, hs, [Lam - (p0 + 2 * i) for i in range(len(hs))], operators=operators)
         return {(c, d): self._output(c, Lam, p0, d, L).apply_raw(nums) for c, d in coords}
 
     def _run(self, batch):

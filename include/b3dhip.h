@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-#define B3D_ABI_VERSION 16
+#define B3D_ABI_VERSION 17
 
 /* Last error message of the calling thread (empty if none). */
 const char* b3d_last_error(void);
@@ -134,10 +134,14 @@ int b3d_moments_stage(int L, const int32_t* rm, const int64_t* re, int64_t rps, 
                       void* stream);
 /* Flush W staged levels m0..m0+W-1 into the moments v (L, nrows, N, R1): v[row, c, k] += sum_w
  * stage[w, row, c] * n_w^k, n_w = m0 + w + srow[row]; NK: exact n^k as KL uint32 limb planes (plane
- * stride nkps, entry n*R1 + k), NKb their bit lengths.  KL <= 8. */
+ * stride nkps, entry n*R1 + k), NKb their bit lengths.  KL <= 8.  Column c = s B + b (structure entry s,
+ * right-hand side b < B, N a multiple of B); jk (Kc codes b R1 + k) limits the flush to those (b, k), e.g.
+ * k <= R_b for right-hand sides of several lambda orders in one recursion (their other moments are neither
+ * read nor written); null: every (b, k < R1), Kc ignored. */
 int b3d_moments_flush(int L, int KL, const int32_t* sm, const int64_t* se, int64_t sps, int W, int m0,
                       const int32_t* srow, const uint32_t* NK, int64_t nkps, const int32_t* NKb, int32_t* vm,
-                      int64_t* ve, int64_t vps, int64_t nrows, int64_t N, int R1, void* stream);
+                      int64_t* ve, int64_t vps, int64_t nrows, int64_t N, int R1, int64_t B, const int32_t* jk,
+                      int Kc, void* stream);
 
 /* Lower-triangular products by exact integer matrices, in the multiprecision GEMM's semantics (column of
  * v aligned to its largest exponent with truncated magnitudes, exact sum, one rounding; the same bits as

@@ -70,3 +70,42 @@ def test_derivs(case):
     bits = -float(mp.log(worst, 2)) if worst else float("inf")
     print(f"{cnt} numerators (R={Rr}, B={B}, kept={c['kept']}): max error relative to the largest coefficient 2^-{bits:.1f}")
     assert bits > 32 * L - 40, bits
+
+
+@pytest.mark.parametrize("L,tj12,tj43,order,Rmax,Bs,window", [(10, 4, 2, 7, 5, [2, 1, 3], 3),
+                                                               (34, 8, 8, 6, 4, [2, 2], 8),
+                                                               (10, 2, 2, 9, 3, [1, 2, 1, 2], 4)])
+def test_orders_in_one_recursion(L, tj12, tj43, order, Rmax, Bs, window):
+    """the right-hand sides of several lambda orders (R = Rmax, Rmax - 2, ...) in ONE recursion with the moment
+    flush limited per column to its order's k <= R (derivatives.run_orders) against one recursion per order:
+    every order's numerators bit for bit, at uneven right-hand-side counts per order"""
+    import torch
+    from blocksvibe_amd import derivatives as DV
+    from blocksvibe_amd import mpf
+    from blocksvibe_amd.derivatives import Derivs
+    from blocksvibe_amd.plan import Plan
+    from blocksvibe_amd.recursion import Recursion
+    pp = ((0, 1), (1, 0))
+    plan = Plan(tj12, tj43, 2 if tj12 == 8 else 6, order, pp, reuse=True)
+    M, H = random_operator(plan, tj12, tj43, pp, order, sum(Bs), 11)
+    h_all = h_tensor(H, plan, L)
+    hs, b0 = [], 0
+    for B in Bs:
+        hs.append(mpf.MPF(h_all.m[..., b0:b0 + B].contiguous(), h_all.e[..., b0:b0 + B].contiguous()))
+        b0 += B
+    Rs = [max(Rmax - 2 * i, 0) for i in range(len(Bs))]
+    rec = Recursion(plan, M, L)
+    der = Derivs(plan, Rmax, L)
+    print(f"orders R={Rs}, right-hand sides {Bs}, L={L}", flush=True)
+    want = []
+    for h, R in zip(hs, Rs):
+        dn = der.for_order(R)
+        dn.start(h.shape[-1], window=window)
+        rec.run(h, hook=dn.accumulate)
+        want.append(dn.finish(h))
+    got = DV.run_orders(rec, der, hs, Rs, window=window)
+    got2 = DV.run_orders(rec, der, hs, Rs, h_all=h_all, window=window)
+    for w, g, g2, R in zip(want, got, got2, Rs):
+        assert w.shape == g.shape == g2.shape, (R, w.shape, g.shape)
+        assert torch.equal(w.m, g.m) and torch.equal(w.e, g.e), ("order", R)
+        assert torch.equal(w.m, g2.m) and torch.equal(w.e, g2.e), ("order, resident h_all", R)
