@@ -198,8 +198,6 @@ class Derivs:
         for jo, (tj, ps) in enumerate(zip(self.tjs, self.poles)):
             didx[jo, :len(ps)] = [di[int(S.pole_delta2(t, k, tj))] for t, k in ps]
         self.didx = torch.from_numpy(didx).to(mpf.dev())
-        self.Cb = mpf.from_pairs([mpf.norm_int(comb(e, d), 0, L) for e in range(R + 1) for d in range(R + 1)],
-                                 L, (R + 1, R + 1))
         # numerator matrix (Jo, Dout, NP + R + 1): columns Pi_i (pole terms), then Pi (x + ub)^t
         mats, npjs = [], []
         for tj in self.tjs:
@@ -350,7 +348,7 @@ class Derivs:
             Gz = mpf.cat([G.reshape(J, Rw * R1, Nc), mpf.zeros((J, 1, Nc), L)], 1)
             Gs = Gz.permute(1, 0, 2).take(gidx).reshape(R1, R1, J, Nc)                  # (s, d, J, Nc)
             Gs = Gs.permute(1, 2, 0, 3).reshape(R1, J * R1 * Nc)                        # (d, J s Nc)
-            Hm = rns.gemm(self.Cb, Gs).reshape(R1, J, R1, Nc)                           # (e, J, s, Nc)
+            Hm = self._binomial_product(Gs).reshape(R1, J, R1, Nc)                      # (e, J, s, Nc)
             Hz = mpf.cat([Hm.permute(0, 2, 1, 3).reshape(R1 * R1, J, Nc), mpf.zeros((1, J, Nc), L)], 0)
             cz = mpf.cat([c[:, j0:j1], mpf.zeros((1, J, Nc), L)], 0)                    # (d | 0, J, Nc)
             cbx = mpf.MPF(cb.m.expand(L, R1 * R1, J, Nc).contiguous(), cb.e.expand(R1 * R1, J, Nc).contiguous())
@@ -362,6 +360,18 @@ class Derivs:
             num = rns.gemm(A.contiguous(), Bn)                                          # (J, Dout, e Nc)
             outs.append(num.reshape(J, self.Dout, R1, 2, S2, B))
         return outs[0] if len(outs) == 1 else mpf.cat(outs, 0)
+
+    def _binomial_product(self, Gs):
+        """Cb Gs for Gs (R + 1, N): the multiprecision GEMM's bits (each column aligned to its largest exponent,
+        exact sum, one rounding) by the triangular product with the exact binomials"""
+        TL, tm, tsg, texp, tsel = self._cbtri
+        R1, N = Gs.shape
+        Gs = Gs.contiguous()
+        out = mpf.empty((R1, N), self.L, Gs.e.device)
+        _lib.call("b3d_tri_product", self.L, TL, Gs.m.data_ptr(), Gs.e.data_ptr(), Gs.numel(), 1, R1, N,
+                  tsel.data_ptr(), tm.data_ptr(), tsg.data_ptr(), tsg.numel(), texp.data_ptr(), out.m.data_ptr(),
+                  out.e.data_ptr(), out.numel(), 1, 0, 0, mpf.stream_ptr())
+        return out
 
     @staticmethod
     def _tri_tables(d2, R1):
@@ -396,6 +406,15 @@ class Derivs:
                              for s in range(R1) for d in range(R1)], device=dev)
         cb = mpf.from_pairs(cbv, L, (R1 * R1, 1, 1))
         self._fc = (jidx, cidx, hidx, gidx, cb)
+        # Hm = Cb Gs by b3d_tri_product: Cb[e][d] = C(e, d) is lower triangular with small exact integer entries
+        # (row unit 2^0), so the triangular product gives the GEMM's bits without the residue conversions
+        ints = [comb(e, d) if d <= e else 0 for e in range(R1) for d in range(R1)]
+        TL = next(t for t in (2, 4, 8) if max(ints).bit_length() <= 32 * t)
+        tm = np.array([[(x >> (32 * l)) & 0xFFFFFFFF for x in ints] for l in range(TL)], dtype=np.uint64)
+        self._cbtri = (TL, torch.from_numpy(tm.astype(np.uint32).view(np.int32).reshape(TL, 1, R1, R1)).to(dev),
+                       torch.zeros((1, R1, R1), dtype=torch.int32, device=dev),
+                       torch.zeros((1, R1), dtype=torch.int64, device=dev),
+                       torch.zeros(1, dtype=torch.int64, device=dev))
 
     def for_order(self, R):
         """the same stages for a smaller R (a higher lambda order of the task): the constants are built once
@@ -428,7 +447,6 @@ class Derivs:
             TL, tm, tsg, texp = self.tri
             d.tri = (TL, tm[:, :, :R1, :R1].contiguous(), tsg[:, :R1, :R1].contiguous(), texp[:, :R1].contiguous())
         d.Wh = mpf.MPF(self.Wh.m[:, :R1].contiguous(), self.Wh.e[:R1].contiguous())
-        d.Cb = mpf.MPF(self.Cb.m[:, :R1, :R1].contiguous(), self.Cb.e[:R1, :R1].contiguous())
         v = [n ** k for n in range(self.NK.shape[1] // (self.R + 1)) for k in range(R1)]
         d.KL = max(1, (max(v).bit_length() + 31) // 32)
         d.NK = torch.from_numpy(np.array([[(x >> (32 * l)) & 0xFFFFFFFF for x in v] for l in range(d.KL)],
