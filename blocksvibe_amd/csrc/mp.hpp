@@ -978,15 +978,16 @@ __device__ __forceinline__ i64 mom_window(const int32_t* __restrict__ sm, const 
   return norm<L, L + 3>(acc, T, r);
 }
 
-// One entry a[e] = sum_{k <= e} T[e][k] v[k] of a lower-triangular product by a matrix of exact integers,
-// with the multiprecision GEMM's semantics: the column v[0..K) aligned to its largest exponent E (magnitudes
-// truncated), the sum of the aligned products exact, one rounding.  T[e][k] = +-tm[k] 2^texp (magnitude of
-// TL limbs at tm planes, entry tb + k, plane stride tps; sign mask tsg[tb + k]): a constant row of exact
-// integers aligned per row is the same integers scaled by a power of two, so the rounded result is
-// identical to the GEMM's.  v[k]: entry vbase + k vstride of the planes vm / ve (plane stride vps).
+// One entry a[e] = sum_{k < kend} T[e][k] v[k] of a product by a matrix of exact integers (kend = e + 1: a
+// lower-triangular one; kend = K: a full row), with the multiprecision GEMM's semantics: the column v[0..K)
+// aligned to its largest exponent E (magnitudes truncated), the sum of the aligned products exact, one
+// rounding.  T[e][k] = +-tm[k] 2^texp (magnitude of TL limbs at tm planes, entry tb + k, plane stride tps;
+// sign mask tsg[tb + k]): a constant row of exact integers aligned per row is the same integers scaled by a
+// power of two, so the rounded result is identical to the GEMM's.  v[k]: entry vbase + k vstride of the planes
+// vm / ve (plane stride vps).
 template <int L, int TL>
 __device__ __forceinline__ i64 tri_entry(const int32_t* __restrict__ vm, const i64* __restrict__ ve, i64 vps,
-                                         i64 vbase, i64 vstride, int K, int e, const int32_t* __restrict__ tm,
+                                         i64 vbase, i64 vstride, int K, int kend, const int32_t* __restrict__ tm,
                                          const uint32_t* __restrict__ tsg, i64 tps, i64 tb, i64 texp,
                                          u32 (&out)[L]) {
   i64 E = B3D_EZ;   // zeros carry EZ, below every exponent
@@ -1000,7 +1001,7 @@ __device__ __forceinline__ i64 tri_entry(const int32_t* __restrict__ vm, const i
   }
   u32 acc[L + TL + 1];
   zero(acc);
-  for (int k = 0; k <= e; ++k) {
+  for (int k = 0; k < kend; ++k) {
     const i64 ek = ve[vbase + k * vstride];
     const u32 st = tsg[tb + k];
     if (ek == B3D_EZ) continue;

@@ -197,6 +197,8 @@ class Derivs:
         didx = np.zeros((Jo, NP), dtype=np.int64)
         for jo, (tj, ps) in enumerate(zip(self.tjs, self.poles)):
             didx[jo, :len(ps)] = [di[int(S.pole_delta2(t, k, tj))] for t, k in ps]
+        self.vand = self._vand_tables([[int(S.pole_delta2(t, k, tj)) for t, k in ps]
+                                       for tj, ps in zip(self.tjs, self.poles)], NP, max(R, 1))
         self.didx = torch.from_numpy(didx).to(mpf.dev())
         # numerator matrix (Jo, Dout, NP + R + 1): columns Pi_i (pole terms), then Pi (x + ub)^t
         mats, npjs = [], []
@@ -327,7 +329,9 @@ class Derivs:
             j1 = min(Jo, j0 + chunk)
             J = j1 - j0
             v = mom[j0:j1].reshape(J * NP, R1, Nc)
-            T = self.Ttab.take(self.didx[j0:j1].reshape(-1))                            # (J NP, e, d)
+            # the GEMM operands of the pole tables, only where a product kernel by exact integers is not taken
+            T = (self.Ttab.take(self.didx[j0:j1].reshape(-1))                           # (J NP, e, d)
+                 if self.tri is None or self.vand is None else None)
             # numerator GEMM operand (J, NP + R1, e Nc): rows < NP the residues a_i[e], then P below
             Bn = mpf.MPF(torch.empty((L, J, NP + R1, R1 * Nc), dtype=torch.int32, device=v.e.device),
                          torch.empty((J, NP + R1, R1 * Nc), dtype=torch.int64, device=v.e.device))
@@ -342,9 +346,12 @@ class Derivs:
                 a = rns.gemm(T, v)                                                      # (J NP, e, Nc)
                 Bn.m[:, :, :NP] = a.m.reshape(L, J, NP, R1 * Nc)
                 Bn.e[:, :NP] = a.e.reshape(J, NP, R1 * Nc)
-            Vand = mpf.MPF(T.m[:, :, :Rw, 0].reshape(L, J, NP, Rw).permute(0, 1, 3, 2).contiguous(),
-                           T.e[:, :Rw, 0].reshape(J, NP, Rw).permute(0, 2, 1).contiguous())  # (J, w, NP)
-            G = rns.gemm(Vand, v.reshape(J, NP, R1 * Nc))                               # (J, w, d Nc)
+            if self.vand is not None:                                                   # (J, w, d Nc)
+                G = self._vand_product(v.reshape(J, NP, R1 * Nc), j0, j1, Rw)
+            else:
+                Vand = mpf.MPF(T.m[:, :, :Rw, 0].reshape(L, J, NP, Rw).permute(0, 1, 3, 2).contiguous(),
+                               T.e[:, :Rw, 0].reshape(J, NP, Rw).permute(0, 2, 1).contiguous())  # (J, w, NP)
+                G = rns.gemm(Vand, v.reshape(J, NP, R1 * Nc))
             Gz = mpf.cat([G.reshape(J, Rw * R1, Nc), mpf.zeros((J, 1, Nc), L)], 1)
             Gs = Gz.permute(1, 0, 2).take(gidx).reshape(R1, R1, J, Nc)                  # (s, d, J, Nc)
             Gs = Gs.permute(1, 2, 0, 3).reshape(R1, J * R1 * Nc)                        # (d, J s Nc)
@@ -360,6 +367,39 @@ class Derivs:
             num = rns.gemm(A.contiguous(), Bn)                                          # (J, Dout, e Nc)
             outs.append(num.reshape(J, self.Dout, R1, 2, S2, B))
         return outs[0] if len(outs) == 1 else mpf.cat(outs, 0)
+
+    @staticmethod
+    def _vand_tables(d2s, NP, Rw):
+        """the powers D_i^w (w < Rw) of each spin's pole positions D_i = d_i / 2 as exact integers d_i^w with the
+        row unit 2^-w, zero for the padding poles, for b3d_int_product: (TL, magnitude limb planes
+        (TL, Jo, Rw, NP), sign masks (Jo, Rw, NP), unit exponents (Jo, Rw)); None if an entry needs more than 8
+        limbs"""
+        Jo = len(d2s)
+        ints = [[(d2[i] ** w if i < len(d2) else 0) for w in range(Rw) for i in range(NP)] for d2 in d2s]
+        bits = max(abs(x).bit_length() for row in ints for x in row)
+        TL = next((t for t in (2, 4, 8) if bits <= 32 * t), None)
+        if TL is None:
+            return None
+        mags = [abs(x) for row in ints for x in row]
+        tm = np.array([[(x >> (32 * l)) & 0xFFFFFFFF for x in mags] for l in range(TL)], dtype=np.uint64)
+        tm = torch.from_numpy(tm.astype(np.uint32).view(np.int32).reshape(TL, Jo, Rw, NP)).to(mpf.dev())
+        tsg = torch.tensor([-1 if x < 0 else 0 for row in ints for x in row], dtype=torch.int32,
+                           device=mpf.dev()).reshape(Jo, Rw, NP)
+        texp = torch.tensor([-w for _ in range(Jo) for w in range(Rw)], dtype=torch.int64,
+                            device=mpf.dev()).reshape(Jo, Rw)
+        return TL, tm, tsg, texp
+
+    def _vand_product(self, v, j0, j1, Rw):
+        """G[j] = (D_i^w)_{w, i} v[j] for the spins j0 .. j1 (v: (J, NP, N)): the GEMM's bits (each column
+        aligned to its largest exponent, exact sum, one rounding) by b3d_int_product with the exact powers"""
+        TL, tm, tsg, texp = self.vand
+        J, NP, N = v.shape
+        v = v.contiguous()
+        out = mpf.empty((J, Rw, N), self.L, v.e.device)
+        _lib.call("b3d_int_product", self.L, TL, v.m.data_ptr(), v.e.data_ptr(), v.numel(), J, Rw, NP, 0, N,
+                  self._jsel[j0:j1].data_ptr(), tm.data_ptr(), tsg.data_ptr(), tsg.numel(), texp.data_ptr(),
+                  out.m.data_ptr(), out.e.data_ptr(), out.numel(), 1, Rw * N, 0, mpf.stream_ptr())
+        return out
 
     def _binomial_product(self, Gs):
         """Cb Gs for Gs (R + 1, N): the multiprecision GEMM's bits (each column aligned to its largest exponent,
@@ -406,6 +446,7 @@ class Derivs:
                              for s in range(R1) for d in range(R1)], device=dev)
         cb = mpf.from_pairs(cbv, L, (R1 * R1, 1, 1))
         self._fc = (jidx, cidx, hidx, gidx, cb)
+        self._jsel = torch.arange(len(self.tjs), dtype=torch.int64, device=dev)      # spin j's table is j
         # Hm = Cb Gs by b3d_tri_product: Cb[e][d] = C(e, d) is lower triangular with small exact integer entries
         # (row unit 2^0), so the triangular product gives the GEMM's bits without the residue conversions
         ints = [comb(e, d) if d <= e else 0 for e in range(R1) for d in range(R1)]
@@ -446,6 +487,10 @@ class Derivs:
         if self.tri is not None:
             TL, tm, tsg, texp = self.tri
             d.tri = (TL, tm[:, :, :R1, :R1].contiguous(), tsg[:, :R1, :R1].contiguous(), texp[:, :R1].contiguous())
+        if self.vand is not None:
+            TL, tm, tsg, texp = self.vand
+            Rw = max(R, 1)
+            d.vand = (TL, tm[:, :, :Rw].contiguous(), tsg[:, :Rw].contiguous(), texp[:, :Rw].contiguous())
         d.Wh = mpf.MPF(self.Wh.m[:, :R1].contiguous(), self.Wh.e[:R1].contiguous())
         v = [n ** k for n in range(self.NK.shape[1] // (self.R + 1)) for k in range(R1)]
         d.KL = max(1, (max(v).bit_length() + 31) // 32)

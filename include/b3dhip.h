@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-#define B3D_ABI_VERSION 17
+#define B3D_ABI_VERSION 18
 
 /* Last error message of the calling thread (empty if none). */
 const char* b3d_last_error(void);
@@ -154,6 +154,16 @@ int b3d_moments_flush(int L, int KL, const int32_t* sm, const int64_t* se, int64
 int b3d_tri_product(int L, int TL, const int32_t* vm, const int64_t* ve, int64_t vps, int64_t nb, int K, int64_t Nc,
                     const int64_t* tsel, const int32_t* tm, const uint32_t* tsg, int64_t tps, const int64_t* texp,
                     int32_t* om, int64_t* oe, int64_t ops, int64_t nper, int64_t obatch, int64_t orow, void* stream);
+
+/* The same product by M x K exact integer matrices with full rows (lower != 0: k <= e only, M <= K):
+ * out(b, e, c) = sum_k T_t[e][k] v(b, k, c), e < M; tables tm (TL, ntab, M, K) (plane stride tps), tsg
+ * (ntab, M, K), texp (ntab, M); v and the output as b3d_tri_product; M, K <= 4096.  Replaces the GEMMs of
+ * reference blocksvibe/derivatives.py:560 (G = the powers D_i^w of the pole positions times the moments)
+ * and :567 (the binomials C(e, d) times G), whose A operands are exact integers with a row unit 2^-w. */
+int b3d_int_product(int L, int TL, const int32_t* vm, const int64_t* ve, int64_t vps, int64_t nb, int M, int K,
+                    int lower, int64_t Nc, const int64_t* tsel, const int32_t* tm, const uint32_t* tsg, int64_t tps,
+                    const int64_t* texp, int32_t* om, int64_t* oe, int64_t ops, int64_t nper, int64_t obatch,
+                    int64_t orow, void* stream);
 
 /* ---- exact multiprecision GEMM through residues on int8 MFMA (replaces the unvendored sdpvibe.mpf.gemm
  *      reached through reference blocksvibe/recursion_gpu.py:53-79 gemm/bgemm, called at

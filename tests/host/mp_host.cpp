@@ -139,7 +139,7 @@ template <int L, int TL> static int64_t tri_t(const int32_t* vm, const int64_t* 
                                               int64_t vstride, int K, int e, const int32_t* tm, const uint32_t* tsg,
                                               int64_t tps, int64_t tb, int64_t texp, uint32_t* r) {
   u32 R[L];
-  const int64_t eo = mp::tri_entry<L, TL>(vm, ve, vps, vbase, vstride, K, e, tm, tsg, tps, tb, texp, R);
+  const int64_t eo = mp::tri_entry<L, TL>(vm, ve, vps, vbase, vstride, K, e + 1, tm, tsg, tps, tb, texp, R);
   copy_out(r, R);
   return eo;
 }
