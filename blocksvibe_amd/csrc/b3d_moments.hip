@@ -99,7 +99,20 @@ int b3d_moments_flush(int L, int KL, const int32_t* sm, const int64_t* se, int64
   if (b3d::check_planes("b3d_moments_flush", {sps, vps})) return 2;
   hipStream_t s = (hipStream_t)stream;
   const int64_t n = nrows * (N / B) * Kc;
-  if (KL <= 4) {
+  // the products y n^k run over exactly the limbs n^k has in the table (KL): no product by a zero limb
+  if (KL == 1) {
+    B3D_DISPATCH_L(L, k_mom_flush<LL, 1><<<grid_for(n, 256), 256, 0, s>>>(sm, se, sps, W, m0, srow, NK, nkps, KL,
+                                                                           NKb, vm, ve, vps, nrows, N, R1, B, jk,
+                                                                           Kc));
+  } else if (KL == 2) {
+    B3D_DISPATCH_L(L, k_mom_flush<LL, 2><<<grid_for(n, 256), 256, 0, s>>>(sm, se, sps, W, m0, srow, NK, nkps, KL,
+                                                                           NKb, vm, ve, vps, nrows, N, R1, B, jk,
+                                                                           Kc));
+  } else if (KL == 3) {
+    B3D_DISPATCH_L(L, k_mom_flush<LL, 3><<<grid_for(n, 256), 256, 0, s>>>(sm, se, sps, W, m0, srow, NK, nkps, KL,
+                                                                           NKb, vm, ve, vps, nrows, N, R1, B, jk,
+                                                                           Kc));
+  } else if (KL == 4) {
     B3D_DISPATCH_L(L, k_mom_flush<LL, 4><<<grid_for(n, 256), 256, 0, s>>>(sm, se, sps, W, m0, srow, NK, nkps, KL,
                                                                            NKb, vm, ve, vps, nrows, N, R1, B, jk,
                                                                            Kc));
