@@ -115,11 +115,8 @@ class Batch:
         pl = group.plan
         self.out = output_map(self.p0, group.L)
         self.ns = list(range(self.p0, LAMBDA + 1, 2))
-        hs = [mpf.random_device((2, ORDER + 1, pl.Jh, pl.S2, NQ_PER_SIGN), group.L,
-                                seed=seed * 100 + 10 * self.p0 + i) for i in range(len(self.ns))]
-        # resident with the lambda orders side by side (one recursion takes the right-hand sides of all of them)
-        self.h_all = mpf.cat(hs, 4)
-        self.hs = [self.h_all[..., i * NQ_PER_SIGN:(i + 1) * NQ_PER_SIGN] for i in range(len(self.ns))]
+        self.hs = [mpf.random_device((2, ORDER + 1, pl.Jh, pl.S2, NQ_PER_SIGN), group.L,
+                                     seed=seed * 100 + 10 * self.p0 + i) for i in range(len(self.ns))]
         self.tasks = []
         for qi, q4 in enumerate(structures(sign)):
             argv = ["--j-external", ",".join(f"{x / 2:g}" for x in TWO_JS), "--j-internal", f"0-{TJMAX // 2}",
@@ -151,17 +148,16 @@ class Batch:
         return pl.S2 * (pl.cauchy_terms() + pole) * len(self.ns)
 
     def run(self, timer=None):
-        """the chain -> write_output result (outputs, Jo, 2, S2, nq, Dmax), on the device: the recursion over
-        the right-hand sides of ORDERS_PER_RUN lambda orders at a time (all of them by default; one recursion,
-        each order's moments and numerators from its own columns)"""
+        """the chain -> write_output result (outputs, Jo, 2, S2, nq, Dmax), on the device: one recursion per
+        ORDERS_PER_RUN lambda orders (one by default: with two lanes the end-to-end step measured the same as with
+        several orders per recursion, at a smaller HBM footprint; 0: all of a sign's orders in one recursion, each
+        order's moments and numerators from its own columns)"""
         from blocksvibe_amd import derivatives as DV
         k = ORDERS_PER_RUN or len(self.ns)
         nums = []
         for i0 in range(0, len(self.ns), k):
             ns, hs = self.ns[i0:i0 + k], self.hs[i0:i0 + k]
-            whole = len(ns) == len(self.ns)
-            nums += DV.run_orders(self.g.rec, self.der, hs, [LAMBDA - n for n in ns], timer=timer,
-                                  h_all=self.h_all if whole else None)
+            nums += DV.run_orders(self.g.rec, self.der, hs, [LAMBDA - n for n in ns], timer=timer)
         return self.out.apply_raw(nums)
 
     def jobs(self):

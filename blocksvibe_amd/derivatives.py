@@ -402,8 +402,8 @@ class Derivs:
         return out
 
     def _binomial_product(self, Gs):
-        """Cb Gs for Gs (R + 1, N): the multiprecision GEMM's bits (each column aligned to its largest exponent,
-        exact sum, one rounding) by the triangular product with the exact binomials"""
+        """C Gs for Gs (R + 1, N), C[e][d] = binomial(e, d): the multiprecision GEMM's bits (each column aligned to its
+        largest exponent, exact sum, one rounding) by the triangular product with the exact binomials"""
         TL, tm, tsg, texp, tsel = self._cbtri
         R1, N = Gs.shape
         Gs = Gs.contiguous()
@@ -447,8 +447,8 @@ class Derivs:
         cb = mpf.from_pairs(cbv, L, (R1 * R1, 1, 1))
         self._fc = (jidx, cidx, hidx, gidx, cb)
         self._jsel = torch.arange(len(self.tjs), dtype=torch.int64, device=dev)      # spin j's table is j
-        # Hm = Cb Gs by b3d_tri_product: Cb[e][d] = C(e, d) is lower triangular with small exact integer entries
-        # (row unit 2^0), so the triangular product gives the GEMM's bits without the residue conversions
+        # Hm = C Gs by b3d_tri_product: C[e][d] = binomial(e, d) is lower triangular with small exact integer
+        # entries (row unit 2^0), so the triangular product gives the GEMM's bits without the residue conversions
         ints = [comb(e, d) if d <= e else 0 for e in range(R1) for d in range(R1)]
         TL = next(t for t in (2, 4, 8) if max(ints).bit_length() <= 32 * t)
         tm = np.array([[(x >> (32 * l)) & 0xFFFFFFFF for x in ints] for l in range(TL)], dtype=np.uint64)
