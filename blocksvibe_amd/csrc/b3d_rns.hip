@@ -277,12 +277,15 @@ __global__ __launch_bounds__(64) void k_crt(const uint16_t* __restrict__ Cr, i64
       int8_t* yrow = yv + lane * YROW;
       double qs = 0.0;
       if (o < total) {
-        for (int t0 = 0; t0 < nmod; t0 += 16) {
-          uint32_t c[16];   // 16 loads in flight per lane
+        // 64 loads in flight per lane: the kernel runs one wave per SIMD (its LDS buffer), so the wave's loads
+        // are the only memory parallelism there is, and its register budget has room for them
+        constexpr int NF = 64;
+        for (int t0 = 0; t0 < nmod; t0 += NF) {
+          uint32_t c[NF];
 #pragma unroll
-          for (int u = 0; u < 16; ++u) c[u] = t0 + u < nmod ? (uint32_t)Cr[(i64)(t0 + u) * total + o] : 0u;
+          for (int u = 0; u < NF; ++u) c[u] = t0 + u < nmod ? (uint32_t)Cr[(i64)(t0 + u) * total + o] : 0u;
 #pragma unroll
-          for (int u = 0; u < 16; ++u) {
+          for (int u = 0; u < NF; ++u) {
             const int t = t0 + u;
             if (t < nmod) {
               // digit pair (h, l) of the balanced y_t: y_t = 256 h + l
