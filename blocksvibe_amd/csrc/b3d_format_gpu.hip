@@ -74,9 +74,22 @@ __global__ __launch_bounds__(256) void k_fmt_sizes(const int32_t* __restrict__ m
     int size = 0;
     i64 mx = -1, emx = 0;
     u32 tmx = 0u;
+    // the exponents are loaded eight coefficients ahead of the comparisons (they do not depend on them), so a
+    // thread waits for one exponent load per group instead of one per coefficient
+    constexpr int G = 8;
+    i64 ev[G];
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+      ev[u] = B3D_EZ;
+      if (deg - 1 - u >= 0) ev[u] = e[base + deg - 1 - u];
+    }
     for (int d = deg - 1; d >= 0; --d) {
       const i64 i = base + d;
-      const i64 ei = e[i];
+      const i64 ei = ev[0];
+#pragma unroll
+      for (int u = 0; u + 1 < G; ++u) ev[u] = ev[u + 1];
+      ev[G - 1] = B3D_EZ;
+      if (d - G >= 0) ev[G - 1] = e[base + d - G];
       if (ei == B3D_EZ) continue;   // canonical zero: 0 < 2^eps 0 is false
       const u32 ti = (u32)m[(i64)(L - 1) * ps + i];
       if (mx < 0 || lt_abs_fast<L>(m, ps, mx, tmx, emx, i, ti, ei + eps)) size = d + 1;
